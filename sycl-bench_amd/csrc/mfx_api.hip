@@ -1,0 +1,522 @@
+// The C ABI of include/mfx.h: argument validation, handle and workspace
+// management, kernel timing, and dispatch to the kernel families.
+#include <cstdio>
+#include <map>
+
+#include "mfx_internal.h"
+
+namespace mfx {
+
+// Kernel-family entry points defined in the other translation units.
+int f32_config_count();
+bool f32_config_info(int c, int* bm, int* bn, int* bk, int* waves);
+int bf_config_count();
+bool bf_config_info(int c, int* bm, int* bn, int* bk, int* waves);
+int fp8_config_count();
+bool fp8_config_info(int c, int* bm, int* bn, int* bk, int* waves);
+mfx_status launch_gemm_f64(Handle* h, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+    int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t s);
+mfx_status launch_convert(Handle* h, int64_t rows, int64_t cols, const float* src, int64_t lds, double* dst,
+    int64_t ldd, hipStream_t s);
+mfx_status launch_rel_frobenius(Handle* h, int uplo, int64_t rows, int64_t cols, const float* X, int64_t ldx,
+    const double* R, int64_t ldr, double* out_host, hipStream_t s);
+mfx_status launch_percent_diff(Handle* h, int64_t rows, int64_t cols, const float* X, int64_t ldx, const double* R,
+    int64_t ldr, double threshold, int64_t* out_host, hipStream_t s);
+mfx_status launch_fill_uniform(Handle* h, int64_t rows, int64_t cols, float* p, int64_t ld, uint64_t seed,
+    uint32_t mid, hipStream_t s);
+mfx_status launch_fill_lognormal(Handle* h, int64_t rows, int64_t cols, float* p, int64_t ld, uint64_t seed,
+    uint32_t mid, float mu, float sigma, hipStream_t s);
+mfx_status launch_split_elementwise(Handle* h, int64_t n, const float* x, uint16_t* hi, uint16_t* lo, hipStream_t s);
+mfx_status launch_quantize_fp8(Handle* h, int64_t rows, int64_t cols, const float* x, int64_t ldx, int block,
+    uint8_t* q, int32_t* exps, hipStream_t s);
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+mfx_status hip_fail(hipError_t e, const char* what) {
+  set_last_error(std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")");
+  if(e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return MFX_ERROR_ALLOC;
+  if(e == hipErrorNoDevice || e == hipErrorInvalidDevice) return MFX_ERROR_NO_DEVICE;
+  return MFX_ERROR_HIP;
+}
+
+mfx_status invalid(const std::string& msg) {
+  set_last_error(msg);
+  return MFX_ERROR_INVALID_VALUE;
+}
+
+mfx_status unsupported(const std::string& msg) {
+  set_last_error(msg);
+  return MFX_ERROR_NOT_SUPPORTED;
+}
+
+hipEvent_t take_event(Handle* h) {
+  if(!h->free_events.empty()) {
+    hipEvent_t e = h->free_events.back();
+    h->free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if(hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+mfx_status workspace(Handle* h, size_t bytes, void** out) {
+  if(bytes <= h->workspace_bytes && h->workspace) {
+    *out = h->workspace;
+    return MFX_SUCCESS;
+  }
+  if(h->capturing) return invalid("workspace must be reserved before graph capture");
+  if(h->workspace) {
+    MFX_HIP(hipDeviceSynchronize());
+    MFX_HIP(hipFree(h->workspace));
+    h->workspace = nullptr;
+    h->workspace_bytes = 0;
+  }
+  const size_t want = (size_t)round_up((int64_t)bytes, 1 << 21);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, want);
+  if(e != hipSuccess) return hip_fail(e, "hipMalloc(workspace)");
+  h->workspace = p;
+  h->workspace_bytes = want;
+  *out = p;
+  return MFX_SUCCESS;
+}
+
+int config_count(mfx_precision p) {
+  switch(p) {
+    case MFX_PREC_FP32: return f32_config_count();
+    case MFX_PREC_SPLIT_BF16:
+    case MFX_PREC_BF16: return bf_config_count();
+    case MFX_PREC_FP8_BLOCK:
+    case MFX_PREC_FP8_BLOCK2: return fp8_config_count();
+  }
+  return 0;
+}
+
+bool config_info(mfx_precision p, int c, int* bm, int* bn, int* bk, int* waves) {
+  switch(p) {
+    case MFX_PREC_FP32: return f32_config_info(c, bm, bn, bk, waves);
+    case MFX_PREC_SPLIT_BF16:
+    case MFX_PREC_BF16: return bf_config_info(c, bm, bn, bk, waves);
+    case MFX_PREC_FP8_BLOCK:
+    case MFX_PREC_FP8_BLOCK2: return fp8_config_info(c, bm, bn, bk, waves);
+  }
+  return false;
+}
+
+// Tuned choices: (precision, shape bucket) -> config.  Buckets are log2 of
+// m, n, k rounded up, so shapes within a factor of two share a decision.
+namespace {
+std::mutex g_tune_mutex;
+std::map<std::string, int> g_tuned;
+
+int log2_bucket(int64_t x) {
+  int b = 0;
+  while((int64_t(1) << b) < x && b < 40) ++b;
+  return b;
+}
+}  // namespace
+
+std::string tune_key(mfx_precision p, int64_t m, int64_t n, int64_t k) {
+  char buf[64];
+  snprintf(buf, sizeof buf, "%d:%d:%d:%d", (int)p, log2_bucket(m), log2_bucket(n), log2_bucket(k));
+  return buf;
+}
+
+void record_tuned(const std::string& key, int config) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  g_tuned[key] = config;
+}
+
+std::map<std::string, int> tuned_snapshot() {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  return g_tuned;
+}
+
+void replace_tuned(const std::map<std::string, int>& t) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  g_tuned = t;
+}
+
+// Heuristic used when nothing is tuned: the largest tile that still gives
+// every CU at least one tile.
+int heuristic_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k) {
+  const int cus = h->cu_count > 0 ? h->cu_count : 256;
+  const int count = config_count(p);
+  int best = count > 0 ? count - 1 : 0;
+  // configs 0 (256x256), 3 (256x128), 1 (128x128), 2 (64x64) in order of size
+  const int order[4] = {0, 3, 1, 2};
+  for(int oi = 0; oi < 4; ++oi) {
+    const int c = order[oi];
+    int bm, bn, bk, w;
+    if(c >= count || !config_info(p, c, &bm, &bn, &bk, &w)) continue;
+    const int64_t tiles = ceil_div(m, bm) * ceil_div(n, bn);
+    if(tiles >= cus) return c;
+    best = c;
+  }
+  (void)k;
+  return best;
+}
+
+int choose_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k) {
+  if(h->forced_config[p] >= 0) return h->forced_config[p];
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    auto it = g_tuned.find(tune_key(p, m, n, k));
+    if(it != g_tuned.end()) return it->second;
+  }
+  return heuristic_config(h, p, m, n, k);
+}
+
+// Square-tile choice for the triangular schedules.
+int choose_square_config(Handle* h, mfx_precision p, int64_t n, int64_t k) {
+  int c = choose_config(h, p, n, n, k);
+  int bm, bn, bk, w;
+  if(config_info(p, c, &bm, &bn, &bk, &w) && bm == bn) return c;
+  return 0;
+}
+
+mfx_status dispatch_dense(Handle* h, mfx_precision prec, DenseOp& op, int config, hipStream_t s) {
+  switch(prec) {
+    case MFX_PREC_FP32: return launch_dense_f32(h, op, config, s);
+    case MFX_PREC_SPLIT_BF16: return launch_dense_bf16x(h, op, true, config, s);
+    case MFX_PREC_BF16: return launch_dense_bf16x(h, op, false, config, s);
+    case MFX_PREC_FP8_BLOCK: return launch_dense_fp8(h, op, false, config, s);
+    case MFX_PREC_FP8_BLOCK2: return launch_dense_fp8(h, op, true, config, s);
+  }
+  return invalid("unknown precision");
+}
+
+bool valid_prec(int p) { return p >= MFX_PREC_FP32 && p <= MFX_PREC_FP8_BLOCK2; }
+
+// Shared validation of one row-major operand view.
+mfx_status check_matrix(const char* name, const void* p, int64_t rows, int64_t cols, int64_t ld) {
+  if(rows < 0 || cols < 0) return invalid(std::string(name) + ": negative dimension");
+  if(rows > 0 && cols > 0 && p == nullptr) return invalid(std::string(name) + ": null pointer");
+  if(ld < std::max<int64_t>(1, cols)) return invalid(std::string(name) + ": leading dimension smaller than row length");
+  if(rows > (int64_t)1 << 30 || cols > (int64_t)1 << 30) return invalid(std::string(name) + ": dimension too large");
+  return MFX_SUCCESS;
+}
+
+}  // namespace mfx
+
+using namespace mfx;
+
+#define HANDLE_OR_FAIL(hh)                                          \
+  if(!(hh)) return invalid("null handle");                          \
+  Handle* h = &(hh)->h;                                             \
+  if(hipSetDevice(h->device) != hipSuccess) return hip_fail(hipErrorInvalidDevice, "hipSetDevice");
+
+extern "C" {
+
+mfx_status mfx_create(mfx_handle* handle, int device) {
+  if(!handle) return invalid("null handle pointer");
+  *handle = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if(e != hipSuccess || n == 0) {
+    set_last_error("no HIP device available");
+    return MFX_ERROR_NO_DEVICE;
+  }
+  if(device < 0 || device >= n) return invalid("device index out of range");
+  MFX_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  MFX_HIP(hipGetDeviceProperties(&prop, device));
+  if(std::string(prop.gcnArchName).rfind("gfx950", 0) != 0) {
+    set_last_error(std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950 only");
+    return MFX_ERROR_NO_DEVICE;
+  }
+  auto* hs = new(std::nothrow) mfx_handle_s;
+  if(!hs) return MFX_ERROR_ALLOC;
+  hs->h.device = device;
+  hs->h.cu_count = prop.multiProcessorCount;
+  hs->h.device_name = prop.name;
+  *handle = hs;
+  return MFX_SUCCESS;
+}
+
+mfx_status mfx_destroy(mfx_handle handle) {
+  if(!handle) return MFX_SUCCESS;
+  Handle* h = &handle->h;
+  hipSetDevice(h->device);
+  hipDeviceSynchronize();
+  for(auto& r : h->records) {
+    hipEventDestroy(r.start);
+    hipEventDestroy(r.stop);
+  }
+  for(auto e : h->free_events) hipEventDestroy(e);
+  if(h->workspace) hipFree(h->workspace);
+  delete handle;
+  return MFX_SUCCESS;
+}
+
+mfx_status mfx_reserve_workspace(mfx_handle handle, size_t bytes) {
+  HANDLE_OR_FAIL(handle);
+  void* p;
+  return workspace(h, bytes, &p);
+}
+
+size_t mfx_gemm_workspace_size(mfx_precision prec, int64_t m, int64_t n, int64_t k) {
+  if(!valid_prec(prec) || m <= 0 || n <= 0 || k <= 0) return 0;
+  DenseOp op{};
+  op.M = (int)m;
+  op.N = (int)n;
+  op.K = (int)k;
+  op.nprod = 1;
+  size_t best = 0;
+  // The largest need over all configurations, so any tuned choice fits.
+  for(int c = 0; c < config_count(prec); ++c) {
+    size_t b = 0;
+    if(prec == MFX_PREC_SPLIT_BF16 || prec == MFX_PREC_BF16)
+      b = dense_bf16x_workspace(op, prec == MFX_PREC_SPLIT_BF16, c);
+    else if(prec == MFX_PREC_FP8_BLOCK || prec == MFX_PREC_FP8_BLOCK2)
+      b = dense_fp8_workspace(op, prec == MFX_PREC_FP8_BLOCK2, c);
+    best = std::max(best, b);
+  }
+  return best;
+}
+
+const char* mfx_status_string(mfx_status status) {
+  switch(status) {
+    case MFX_SUCCESS: return "MFX_SUCCESS";
+    case MFX_ERROR_INVALID_VALUE: return "MFX_ERROR_INVALID_VALUE";
+    case MFX_ERROR_NOT_SUPPORTED: return "MFX_ERROR_NOT_SUPPORTED";
+    case MFX_ERROR_HIP: return "MFX_ERROR_HIP";
+    case MFX_ERROR_ALLOC: return "MFX_ERROR_ALLOC";
+    case MFX_ERROR_NO_DEVICE: return "MFX_ERROR_NO_DEVICE";
+    case MFX_ERROR_TIMEOUT: return "MFX_ERROR_TIMEOUT";
+    case MFX_ERROR_IO: return "MFX_ERROR_IO";
+  }
+  return "MFX_ERROR_UNKNOWN";
+}
+
+const char* mfx_last_error(void) { return g_last_error.c_str(); }
+
+mfx_status mfx_device_name(mfx_handle handle, char* buf, size_t len) {
+  HANDLE_OR_FAIL(handle);
+  if(!buf || len == 0) return invalid("null buffer");
+  snprintf(buf, len, "%s", h->device_name.c_str());
+  return MFX_SUCCESS;
+}
+
+mfx_status mfx_gemm(mfx_handle handle, mfx_precision prec, mfx_op transA, mfx_op transB, int64_t m, int64_t n,
+    int64_t k, float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc,
+    mfx_stream stream) {
+  HANDLE_OR_FAIL(handle);
+  if(!valid_prec(prec)) return invalid("unknown precision");
+  if((transA != MFX_OP_N && transA != MFX_OP_T) || (transB != MFX_OP_N && transB != MFX_OP_T))
+    return invalid("transA / transB must be MFX_OP_N or MFX_OP_T");
+  if(m < 0 || n < 0 || k < 0) return invalid("negative dimension");
+  MFX_CHECK(transA == MFX_OP_N ? check_matrix("A", A, m, k, lda) : check_matrix("A", A, k, m, lda));
+  MFX_CHECK(transB == MFX_OP_N ? check_matrix("B", B, k, n, ldb) : check_matrix("B", B, n, k, ldb));
+  MFX_CHECK(check_matrix("C", C, m, n, ldc));
+  if(m == 0 || n == 0) return MFX_SUCCESS;
+  DenseOp op{};
+  op.M = (int)m;
+  op.N = (int)n;
+  op.K = (int)k;
+  op.A = A;
+  op.lda = lda;
+  op.a_kmajor = transA == MFX_OP_N;
+  op.B = B;
+  op.ldb = ldb;
+  op.b_kmajor = transB == MFX_OP_T;
+  op.nprod = 1;
+  op.tiles = TILES_FULL;
+  op.ep = Epilogue{C, ldc, alpha, beta, (int)m, (int)n, TILES_FULL, 0};
+  if(k == 0) {
+    // C = beta * C (alpha * empty sum): the contraction loop runs zero times.
+    op.K = 0;
+  }
+  const int cfg = choose_config(h, prec, m, n, k);
+  return dispatch_dense(h, prec, op, cfg, static_cast<hipStream_t>(stream));
+}
+
+static mfx_status sym_common(Handle* h, mfx_precision prec, mfx_uplo uplo, int mirror, int64_t n, int64_t k,
+    float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc,
+    int nprod, hipStream_t s) {
+  if(!valid_prec(prec)) return invalid("unknown precision");
+  if(uplo != MFX_LOWER && uplo != MFX_UPPER && uplo != MFX_FULL) return invalid("uplo must be LOWER, UPPER or FULL");
+  if(n < 0 || k < 0) return invalid("negative dimension");
+  MFX_CHECK(check_matrix("A", A, n, k, lda));
+  if(nprod == 2) MFX_CHECK(check_matrix("B", B, n, k, ldb));
+  MFX_CHECK(check_matrix("C", C, n, n, ldc));
+  if(n == 0) return MFX_SUCCESS;
+  DenseOp op{};
+  op.M = op.N = (int)n;
+  op.K = (int)k;
+  op.A = A;
+  op.lda = lda;
+  op.a_kmajor = true;
+  op.B = nprod == 2 ? B : A;
+  op.ldb = nprod == 2 ? ldb : lda;
+  op.b_kmajor = true;
+  op.nprod = nprod;
+  const int mode = uplo == MFX_LOWER ? TILES_LOWER : uplo == MFX_UPPER ? TILES_UPPER : TILES_FULL;
+  op.tiles = mode;
+  op.ep = Epilogue{C, ldc, alpha, beta, (int)n, (int)n, mode, mode == TILES_FULL ? 0 : (mirror ? 1 : 0)};
+  const int cfg = mode == TILES_FULL ? choose_config(h, prec, n, n, k * nprod) : choose_square_config(h, prec, n, k * nprod);
+  return dispatch_dense(h, prec, op, cfg, s);
+}
+
+mfx_status mfx_syrk(mfx_handle handle, mfx_precision prec, mfx_uplo uplo, int mirror, int64_t n, int64_t k, float alpha,
+    const float* A, int64_t lda, float beta, float* C, int64_t ldc, mfx_stream stream) {
+  HANDLE_OR_FAIL(handle);
+  return sym_common(h, prec, uplo, mirror, n, k, alpha, A, lda, nullptr, 0, beta, C, ldc, 1,
+      static_cast<hipStream_t>(stream));
+}
+
+mfx_status mfx_syr2k(mfx_handle handle, mfx_precision prec, mfx_uplo uplo, int mirror, int64_t n, int64_t k,
+    float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc,
+    mfx_stream stream) {
+  HANDLE_OR_FAIL(handle);
+  return sym_common(h, prec, uplo, mirror, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 2,
+      static_cast<hipStream_t>(stream));
+}
+
+mfx_status mfx_gemm_f64(mfx_handle handle, mfx_op transA, mfx_op transB, int64_t m, int64_t n, int64_t k, double alpha,
+    const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+    mfx_stream stream) {
+  HANDLE_OR_FAIL(handle);
+  if((transA != MFX_OP_N && transA != MFX_OP_T) || (transB != MFX_OP_N && transB != MFX_OP_T))
+    return invalid("transA / transB must be MFX_OP_N or MFX_OP_T");
+  if(m < 0 || n < 0 || k < 0) return invalid("negative dimension");
+  MFX_CHECK(transA == MFX_OP_N ? check_matrix("A", A, m, k, lda) : check_matrix("A", A, k, m, lda));
+  MFX_CHECK(transB == MFX_OP_N ? check_matrix("B", B, k, n, ldb) : check_matrix("B", B, n, k, ldb));
+  MFX_CHECK(check_matrix("C", C, m, n, ldc));
+  return launch_gemm_f64(h, transA == MFX_OP_T, transB == MFX_OP_T, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+      static_cast<hipStream_t>(stream));
+}
+
+mfx_status mfx_convert_f32_f64(mfx_handle handle, int64_t rows, int64_t cols, const float* src, int64_t lds,
+    double* dst, int64_t ldd, mfx_stream stream) {
+  HANDLE_OR_FAIL(handle);
+  MFX_CHECK(check_matrix("src", src, rows, cols, lds));
+  MFX_CHECK(check_matrix("dst", dst, rows, cols, ldd));
+  return launch_convert(h, rows, cols, src, lds, dst, ldd, static_cast<hipStream_t>(stream));
+}
+
+mfx_status mfx_rel_frobenius(mfx_handle handle, mfx_uplo uplo, int64_t rows, int64_t cols, const float* X, int64_t ldx,
+    const double* R, int64_t ldr, double* out, mfx_stream stream) {
+  HANDLE_OR_FAIL(handle);
+  if(!out) return invalid("null output pointer");
+  if(uplo != MFX_LOWER && uplo != MFX_UPPER && uplo != MFX_FULL) return invalid("bad uplo");
+  MFX_CHECK(check_matrix("X", X, rows, cols, ldx));
+  MFX_CHECK(check_matrix("R", R, rows, cols, ldr));
+  return launch_rel_frobenius(h, uplo, rows, cols, X, ldx, R, ldr, out, static_cast<hipStream_t>(stream));
+}
+
+mfx_status mfx_percent_diff_count(mfx_handle handle, int64_t rows, int64_t cols, const float* X, int64_t ldx,
+    const double* R, int64_t ldr, double threshold, int64_t* out, mfx_stream stream) {
+  HANDLE_OR_FAIL(handle);
+  if(!out) return invalid("null output pointer");
+  MFX_CHECK(check_matrix("X", X, rows, cols, ldx));
+  MFX_CHECK(check_matrix("R", R, rows, cols, ldr));
+  return launch_percent_diff(h, rows, cols, X, ldx, R, ldr, threshold, out, static_cast<hipStream_t>(stream));
+}
+
+mfx_status mfx_fill_uniform(mfx_handle handle, int64_t rows, int64_t cols, float* p, int64_t ld, uint64_t seed,
+    uint32_t matrix_id, mfx_stream stream) {
+  HANDLE_OR_FAIL(handle);
+  MFX_CHECK(check_matrix("p", p, rows, cols, ld));
+  return launch_fill_uniform(h, rows, cols, p, ld, seed, matrix_id, static_cast<hipStream_t>(stream));
+}
+
+mfx_status mfx_fill_lognormal(mfx_handle handle, int64_t rows, int64_t cols, float* p, int64_t ld, uint64_t seed,
+    uint32_t matrix_id, float mu, float sigma, mfx_stream stream) {
+  HANDLE_OR_FAIL(handle);
+  MFX_CHECK(check_matrix("p", p, rows, cols, ld));
+  if(!(sigma >= 0.0f)) return invalid("sigma must be >= 0");
+  return launch_fill_lognormal(h, rows, cols, p, ld, seed, matrix_id, mu, sigma, static_cast<hipStream_t>(stream));
+}
+
+mfx_status mfx_split_bf16(mfx_handle handle, int64_t count, const float* x, uint16_t* hi, uint16_t* lo,
+    mfx_stream stream) {
+  HANDLE_OR_FAIL(handle);
+  if(count < 0) return invalid("negative count");
+  if(count > 0 && (!x || !hi || !lo)) return invalid("null pointer");
+  return launch_split_elementwise(h, count, x, hi, lo, static_cast<hipStream_t>(stream));
+}
+
+mfx_status mfx_quantize_fp8_block(mfx_handle handle, int64_t rows, int64_t cols, const float* x, int64_t ldx,
+    int32_t block, uint8_t* q, int32_t* exps, mfx_stream stream) {
+  HANDLE_OR_FAIL(handle);
+  MFX_CHECK(check_matrix("x", x, rows, cols, ldx));
+  if(block != 128 && block != 64 && block != 32) return invalid("block must be 32, 64 or 128");
+  if(rows > 0 && cols > 0 && (!q || !exps)) return invalid("null output pointer");
+  return launch_quantize_fp8(h, rows, cols, x, ldx, block, q, exps, static_cast<hipStream_t>(stream));
+}
+
+mfx_status mfx_timing_begin(mfx_handle handle) {
+  HANDLE_OR_FAIL(handle);
+  for(auto& r : h->records) {
+    h->free_events.push_back(r.start);
+    h->free_events.push_back(r.stop);
+  }
+  h->records.clear();
+  h->timing = true;
+  return MFX_SUCCESS;
+}
+
+mfx_status mfx_timing_end(mfx_handle handle, int32_t cap, const char** names, int64_t* launches, double* total_ms,
+    int32_t* n_out) {
+  HANDLE_OR_FAIL(handle);
+  h->timing = false;
+  std::vector<const char*> order;
+  std::map<std::string, std::pair<int64_t, double>> agg;
+  for(auto& r : h->records) {
+    MFX_HIP(hipEventSynchronize(r.stop));
+    float ms = 0.0f;
+    MFX_HIP(hipEventElapsedTime(&ms, r.start, r.stop));
+    auto it = agg.find(r.name);
+    if(it == agg.end()) {
+      order.push_back(r.name);
+      agg[r.name] = {1, (double)ms};
+    } else {
+      it->second.first += 1;
+      it->second.second += ms;
+    }
+  }
+  for(auto& r : h->records) {
+    h->free_events.push_back(r.start);
+    h->free_events.push_back(r.stop);
+  }
+  h->records.clear();
+  const int n = (int)order.size();
+  if(n_out) *n_out = n;
+  for(int i = 0; i < n && i < cap; ++i) {
+    auto& v = agg[order[i]];
+    if(names) names[i] = order[i];
+    if(launches) launches[i] = v.first;
+    if(total_ms) total_ms[i] = v.second;
+  }
+  return MFX_SUCCESS;
+}
+
+int32_t mfx_config_count(mfx_precision prec) { return valid_prec(prec) ? config_count(prec) : 0; }
+
+mfx_status mfx_config_info(mfx_precision prec, int32_t config, int32_t* bm, int32_t* bn, int32_t* bk, int32_t* waves) {
+  if(!valid_prec(prec)) return invalid("unknown precision");
+  if(!bm || !bn || !bk || !waves) return invalid("null output pointer");
+  int a, b, c, d;
+  if(!config_info(prec, config, &a, &b, &c, &d)) return invalid("configuration out of range");
+  *bm = a;
+  *bn = b;
+  *bk = c;
+  *waves = d;
+  return MFX_SUCCESS;
+}
+
+mfx_status mfx_force_config(mfx_handle handle, mfx_precision prec, int32_t config) {
+  HANDLE_OR_FAIL(handle);
+  if(!valid_prec(prec)) return invalid("unknown precision");
+  if(config < -1 || config >= config_count(prec)) return invalid("configuration out of range");
+  h->forced_config[prec] = config;
+  return MFX_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,289 @@
+// Internal declarations shared by the libmfx translation units: the handle,
+// launch bookkeeping, number formats (BF16 / OCP e4m3 / keyed RNG) and the
+// tile-scheduling helpers used by every kernel family.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mfx.h"
+
+namespace mfx {
+
+// ---------------------------------------------------------------------------
+// Error plumbing
+// ---------------------------------------------------------------------------
+void set_last_error(const std::string& msg);
+mfx_status hip_fail(hipError_t e, const char* what);
+
+#define MFX_HIP(expr)                                                   \
+  do {                                                                  \
+    hipError_t _e = (expr);                                             \
+    if(_e != hipSuccess) return ::mfx::hip_fail(_e, #expr);             \
+  } while(0)
+
+#define MFX_CHECK(expr)                                                 \
+  do {                                                                  \
+    mfx_status _s = (expr);                                             \
+    if(_s != MFX_SUCCESS) return _s;                                    \
+  } while(0)
+
+mfx_status invalid(const std::string& msg);
+mfx_status unsupported(const std::string& msg);
+
+// ---------------------------------------------------------------------------
+// Handle
+// ---------------------------------------------------------------------------
+struct TimingRecord {
+  const char* name;
+  hipEvent_t start;
+  hipEvent_t stop;
+};
+
+struct Handle {
+  int device = 0;
+  int cu_count = 0;
+  std::string device_name;
+  void* workspace = nullptr;
+  size_t workspace_bytes = 0;
+  bool timing = false;
+  std::vector<TimingRecord> records;
+  std::vector<hipEvent_t> free_events;
+  int forced_config[5] = {-1, -1, -1, -1, -1};
+  bool capturing = false;  // set while a grouped launch is being captured
+};
+
+// Returns a workspace pointer of at least `bytes` (grows the arena; growing
+// synchronises the device, so it must not happen under graph capture).
+mfx_status workspace(Handle* h, size_t bytes, void** out);
+
+// Records an event pair around `launch()` when timing is on, and converts a
+// launch error into MFX_ERROR_HIP.
+template <class F>
+mfx_status timed_launch(Handle* h, const char* name, hipStream_t s, F&& launch);
+
+hipEvent_t take_event(Handle* h);
+
+template <class F>
+mfx_status timed_launch(Handle* h, const char* name, hipStream_t s, F&& launch) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if(h->timing) {
+    a = take_event(h);
+    b = take_event(h);
+    if(!a || !b) return hip_fail(hipErrorOutOfMemory, "hipEventCreate");
+    MFX_HIP(hipEventRecord(a, s));
+  }
+  launch();
+  hipError_t e = hipGetLastError();
+  if(e != hipSuccess) return hip_fail(e, name);
+  if(h->timing) {
+    MFX_HIP(hipEventRecord(b, s));
+    h->records.push_back({name, a, b});
+  }
+  return MFX_SUCCESS;
+}
+
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+inline int64_t ceil_div(int64_t x, int64_t m) { return (x + m - 1) / m; }
+
+// ---------------------------------------------------------------------------
+// Number formats (host + device, bit-exact on both sides)
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint32_t f32_bits(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  return u;
+}
+__host__ __device__ inline float bits_f32(uint32_t u) {
+  float x;
+  memcpy(&x, &u, 4);
+  return x;
+}
+
+// Round-to-nearest-even FP32 -> BF16 (NaN stays NaN, quiet).
+__host__ __device__ inline uint16_t bf16_rne(float x) {
+  uint32_t u = f32_bits(x);
+  if((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf16_to_f32(uint16_t h) { return bits_f32((uint32_t)h << 16); }
+
+// OCP e4m3 (E4M3FN: bias 7, max 448, no infinities, S.1111.111 = NaN),
+// round-to-nearest-even, saturating to +-448.
+__host__ __device__ inline uint8_t e4m3_rne_sat(float x) {
+  uint32_t u = f32_bits(x);
+  uint8_t sign = (uint8_t)((u >> 24) & 0x80u);
+  uint32_t au = u & 0x7fffffffu;
+  if(au > 0x7f800000u) return (uint8_t)(sign | 0x7f);
+  float a = bits_f32(au);
+  if(a > 448.0f) return (uint8_t)(sign | 0x7e);
+  if(a < 0.015625f) {  // below the smallest normal 2^-6: multiples of 2^-9
+    float s = a * 512.0f;  // exact
+    // round half to even on an exact value in [0, 8)
+    uint32_t i = (uint32_t)s;
+    float frac = s - (float)i;
+    if(frac > 0.5f || (frac == 0.5f && (i & 1u))) ++i;
+    return (uint8_t)(sign | i);  // i == 8 encodes the smallest normal
+  }
+  int e = (int)((au >> 23) & 0xffu) - 127;  // -6 .. 8
+  uint32_t m = au & 0x7fffffu;
+  uint32_t r = (m + 0x7ffffu + ((m >> 20) & 1u)) >> 20;  // 0 .. 8
+  uint32_t code = ((uint32_t)(e + 7) << 3) + r;
+  if(code > 0x7eu) code = 0x7eu;
+  return (uint8_t)(sign | code);
+}
+__host__ __device__ inline float e4m3_to_f32(uint8_t q) {
+  uint32_t s = q & 0x80u, e = (q >> 3) & 0xfu, m = q & 7u;
+  float v;
+  if(e == 15 && m == 7) return bits_f32(0x7fc00000u | (s << 24));
+  if(e == 0)
+    v = (float)m * 0.001953125f;  // m * 2^-9
+  else
+    v = bits_f32(((e + 120u) << 23) | (m << 20));
+  return s ? -v : v;
+}
+
+// Smallest e with amax <= 448 * 2^e (0 for amax == 0), clamped to the E8M0
+// range usable as an MFMA scale.
+__host__ __device__ inline int fp8_block_exponent(float amax) {
+  if(!(amax > 0.0f)) return 0;
+  int e = (int)((f32_bits(amax) >> 23) & 0xffu) - 127 - 8;
+  if(((f32_bits(amax) >> 23) & 0xffu) == 0) e = -126 - 8;
+  // adjust so that 448*2^(e-1) < amax <= 448*2^e, using exact power-of-two scaling
+  for(int it = 0; it < 4; ++it) {
+    if(amax > ldexpf(448.0f, e)) ++e;
+    else if(amax <= ldexpf(448.0f, e - 1)) --e;
+    else break;
+  }
+  if(e < -127) e = -127;
+  if(e > 127) e = 127;
+  return e;
+}
+
+// Keyed SplitMix64: the value for (seed, matrix, index) is independent of
+// launch shape, so host and device generate bit-identical inputs.
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline uint64_t keyed_hash(uint64_t seed, uint32_t matrix, uint64_t index) {
+  uint64_t base = mix64(seed ^ (0x9e3779b97f4a7c15ull * ((uint64_t)matrix + 1ull)));
+  return mix64(base + 0x9e3779b97f4a7c15ull * (index + 1ull));
+}
+// 2 * (h >> 40) * 2^-24 - 1: a 24-bit grid on [-1, 1), exact in FP32.
+__host__ __device__ inline float uniform_pm1(uint64_t h) {
+  return (float)(uint32_t)(h >> 40) * 1.1920928955078125e-07f - 1.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Tile scheduling
+// ---------------------------------------------------------------------------
+// Workgroups are dispatched round-robin over the 8 XCDs (wg i -> XCD i % 8).
+// This renumbers them so each XCD owns one contiguous range of tile ids, which
+// keeps the A/B panels a tile needs in that XCD's L2.
+__device__ inline int xcd_remap(int bid, int nblocks) {
+  const int xcd = bid & 7, local = bid >> 3;
+  const int per = nblocks >> 3, rem = nblocks & 7;
+  return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+// Grouped ordering: GROUP consecutive tile rows are walked column by column.
+__device__ inline void grouped_tile(int pid, int tiles_m, int tiles_n, int group, int& tm, int& tn) {
+  const int in_group = group * tiles_n;
+  const int gid = pid / in_group;
+  const int first_m = gid * group;
+  const int gsize = min(tiles_m - first_m, group);
+  const int r = pid - gid * in_group;
+  tm = first_m + r % gsize;
+  tn = r / gsize;
+}
+
+// Tile (ti, tj) with tj <= ti for linear index p over the lower triangle.
+__device__ inline void lower_tile(int p, int& ti, int& tj) {
+  int i = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+  while(i * (i + 1) / 2 > p) --i;
+  while((i + 1) * (i + 2) / 2 <= p) ++i;
+  ti = i;
+  tj = p - i * (i + 1) / 2;
+}
+
+// Scheduling modes shared by the FP32 and BF16 kernel families.
+enum TileMode : int { TILES_FULL = 0, TILES_LOWER = 1, TILES_UPPER = 2 };
+
+// Epilogue parameters common to all families.
+struct Epilogue {
+  float* C;
+  int64_t ldc;
+  float alpha, beta;
+  int M, N;
+  int uplo;    // TILES_FULL: write everything; LOWER/UPPER: that triangle only
+  int mirror;  // also write the opposite triangle from the transposed value
+};
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// Writes one 32x32 MFMA accumulator block whose top-left corner is (r0, c0).
+// v_mfma_f32_32x32x* layout: lane l, register q holds
+// row 8*(q/4) + 4*(l/32) + q%4, column l%32.
+__device__ inline void store_block32(const Epilogue& ep, const floatx16& acc, int r0, int c0, int lane) {
+  const int col = c0 + (lane & 31);
+  const int rbase = r0 + 4 * (lane >> 5);
+#pragma unroll
+  for(int q = 0; q < 16; ++q) {
+    const int row = rbase + 8 * (q >> 2) + (q & 3);
+    if(row >= ep.M || col >= ep.N) continue;
+    bool direct = true;
+    if(ep.uplo == TILES_LOWER) direct = row >= col;
+    else if(ep.uplo == TILES_UPPER) direct = row <= col;
+    if(direct) {
+      float* c = ep.C + (int64_t)row * ep.ldc + col;
+      float v = ep.alpha * acc[q];
+      if(ep.beta != 0.0f) v += ep.beta * *c;
+      *c = v;
+    }
+    if(ep.mirror && ep.uplo != TILES_FULL && row != col && direct) {
+      float* c = ep.C + (int64_t)col * ep.ldc + row;
+      float v = ep.alpha * acc[q];
+      if(ep.beta != 0.0f) v += ep.beta * *c;
+      *c = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers implemented by the kernel translation units
+// ---------------------------------------------------------------------------
+struct DenseOp {
+  // C = alpha * sum_p opA_p * opB_p + beta * C, with p over one or two products
+  // (two for syr2k: A*B^T then B*A^T).
+  int M, N, K;
+  const float* A; int64_t lda; bool a_kmajor;  // element (m,k) at A[m*lda+k] if kmajor else A[k*lda+m]
+  const float* B; int64_t ldb; bool b_kmajor;  // element (k,n) at B[n*ldb+k] if kmajor else B[k*ldb+n]
+  int nprod;                                    // 1, or 2 with operands swapped for the second
+  Epilogue ep;
+  int tiles;                                    // TILES_FULL / LOWER / UPPER
+};
+
+mfx_status launch_dense_f32(Handle* h, const DenseOp& op, int config, hipStream_t s);
+mfx_status launch_dense_bf16x(Handle* h, const DenseOp& op, bool split, int config, hipStream_t s);
+mfx_status launch_dense_fp8(Handle* h, const DenseOp& op, bool two_term, int config, hipStream_t s);
+size_t dense_bf16x_workspace(const DenseOp& op, bool split, int config);
+size_t dense_fp8_workspace(const DenseOp& op, bool two_term, int config);
+
+int config_count(mfx_precision p);
+bool config_info(mfx_precision p, int c, int* bm, int* bn, int* bk, int* waves);
+int choose_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k);
+
+}  // namespace mfx
+
+struct mfx_handle_s {
+  mfx::Handle h;
+};

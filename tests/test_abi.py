@@ -1,0 +1,76 @@
+"""The C-ABI library loads and exports exactly what include/mfx.h declares (CPU only)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mfx.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mfx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for required in ("mfx_gemm", "mfx_syrk", "mfx_syr2k", "mfx_gemm_grouped", "mfx_chain", "mfx_gemm_f64",
+                     "mfx_rel_frobenius", "mfx_autotune", "mfx_status_string", "mfx_timing_begin", "mfx_timing_end"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol(mfx):
+    lib = mfx.load()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert missing == []
+
+
+def test_binding_covers_every_declared_symbol(mfx):
+    assert sorted(mfx.SIGNATURES) == declared_functions()
+
+
+def test_status_strings(mfx):
+    lib = mfx.load()
+    for code, name in enumerate(mfx.STATUS):
+        assert lib.mfx_status_string(code).decode() == name
+    assert lib.mfx_status_string(99).decode() == "MFX_ERROR_UNKNOWN"
+
+
+def test_config_tables(mfx):
+    for prec in ("fp32", "split-bf16", "bf16"):
+        n = mfx.config_count(prec)
+        assert n >= 3
+        for c in range(n):
+            bm, bn, bk, waves = mfx.config_info(prec, c)
+            assert bm % 32 == 0 and bn % 32 == 0 and bk % 2 == 0 and waves >= 1
+    with pytest.raises(mfx.MfxError):
+        mfx.config_info("fp32", 99)
+
+
+def test_workspace_size_is_positive_for_packed_precisions(mfx):
+    lib = mfx.load()
+    assert lib.mfx_gemm_workspace_size(mfx.FP32, 1024, 1024, 1024) == 0
+    ws = lib.mfx_gemm_workspace_size(mfx.SPLIT_BF16, 1000, 1000, 1000)
+    # hi + lo BF16 copies of both padded operands
+    assert ws >= 2 * 2 * 2 * 1000 * 1000
+    assert lib.mfx_gemm_workspace_size(mfx.BF16, 1000, 1000, 1000) < ws
+
+
+def test_create_without_gpu_reports_no_device(mfx):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    lib = mfx.load()
+    h = ctypes.c_void_p()
+    assert lib.mfx_create(ctypes.byref(h), 0) == 5  # MFX_ERROR_NO_DEVICE
+    assert lib.mfx_last_error()
+
+
+def test_null_handle_is_rejected(mfx):
+    lib = mfx.load()
+    st = lib.mfx_gemm(None, 0, 0, 0, 4, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 4, None)
+    assert st == 1  # MFX_ERROR_INVALID_VALUE
+    assert b"null handle" in lib.mfx_last_error()
