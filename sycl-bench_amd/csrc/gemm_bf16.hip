@@ -25,6 +25,9 @@ namespace mfx {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+// Native clang vector (not HIP's uint4 struct) so register arrays of chunks are
+// promoted to registers instead of living in scratch memory.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int KB_PER_STAGE = 2;  // BK = 32
 
@@ -87,8 +90,8 @@ __global__ __launch_bounds__(256) void pack_bf16x_kernel(const PackParams p) {
 // Contract
 // ---------------------------------------------------------------------------
 struct BfParams {
-  const uint4* PA;  // packed A-role operand of product 0
-  const uint4* PB;  // packed B-role operand of product 0
+  const u32x4* PA;  // packed A-role operand of product 0
+  const u32x4* PB;  // packed B-role operand of product 0
   int kbs;          // k blocks of 16 in both packed operands
   int nprod;        // 2: second product uses (PB, PA)
   int tiles_m, tiles_n, group;
@@ -106,14 +109,14 @@ struct BfCfg {
   static_assert(CH_A % NT == 0 && CH_B % NT == 0, "stage must split evenly");
 };
 
-__device__ inline bf16x8 as_bf16x8(const uint4& u) { return __builtin_bit_cast(bf16x8, u); }
+__device__ __forceinline__ bf16x8 as_bf16x8(u32x4 u) { return __builtin_bit_cast(bf16x8, u); }
 
 template <int BM, int BN, int WM, int WN, int NHL>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16x_kernel(const BfParams p) {
   using Cfg = BfCfg<BM, BN, WM, WN, NHL>;
   constexpr int NT = Cfg::NT, FM = Cfg::FM, FN = Cfg::FN;
   constexpr int SA = Cfg::CH_A, SB = Cfg::CH_B;
-  __shared__ uint4 lds[2 * (SA + SB)];
+  __shared__ u32x4 lds[2 * (SA + SB)];
 
   int tm, tn;
   const int pid = xcd_remap(blockIdx.x, gridDim.x);
@@ -141,12 +144,12 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16x_kernel(
   const int KT = p.kbs / KB_PER_STAGE;
   const int steps = KT * p.nprod;
   const int64_t blk_stride = (int64_t)p.kbs * NHL * 64;
-  uint4 ra[Cfg::PER_A], rb[Cfg::PER_B];
+  u32x4 ra[Cfg::PER_A], rb[Cfg::PER_B];
 
-  auto issue = [&](int s) {
+  auto issue = [&](int s) __attribute__((always_inline)) {
     const int prod = s / KT, kt = s - prod * KT;
-    const uint4* A = prod ? p.PB : p.PA;
-    const uint4* B = prod ? p.PA : p.PB;
+    const u32x4* A = prod ? p.PB : p.PA;
+    const u32x4* B = prod ? p.PA : p.PB;
     const int64_t koff = (int64_t)kt * RUN;
 #pragma unroll
     for(int i = 0; i < Cfg::PER_A; ++i) {
@@ -161,26 +164,17 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16x_kernel(
       rb[i] = B[(int64_t)(tn * (BN / 32) + blk) * blk_stride + koff + run];
     }
   };
-  auto commit = [&](int buf) {
-    uint4* sa = lds + buf * (SA + SB);
-    uint4* sb = sa + SA;
+  auto commit = [&](int buf) __attribute__((always_inline)) {
+    u32x4* sa = lds + buf * (SA + SB);
+    u32x4* sb = sa + SA;
 #pragma unroll
     for(int i = 0; i < Cfg::PER_A; ++i) sa[tid + i * NT] = ra[i];
 #pragma unroll
     for(int i = 0; i < Cfg::PER_B; ++i) sb[tid + i * NT] = rb[i];
   };
-
-  if(steps > 0) {
-    issue(0);
-    commit(0);
-  }
-  __syncthreads();
-
-  for(int s = 0; s < steps; ++s) {
-    const int cur = s & 1;
-    if(s + 1 < steps) issue(s + 1);
-    const uint4* sa = lds + cur * (SA + SB);
-    const uint4* sb = sa + SA;
+  auto compute = [&](int cur) __attribute__((always_inline)) {
+    const u32x4* sa = lds + cur * (SA + SB);
+    const u32x4* sb = sa + SA;
 #pragma unroll
     for(int kb = 0; kb < KB_PER_STAGE; ++kb) {
       bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
@@ -214,15 +208,27 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16x_kernel(
 #pragma unroll
         for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
     }
-    if(s + 1 < steps) commit(cur ^ 1);
+  };
+
+  // Steady state without branches around the prefetch, so the registers
+  // holding stage s+1 stay live across the MFMAs of stage s (no early wait).
+  if(steps > 0) {
+    issue(0);
+    commit(0);
+  }
+  __syncthreads();
+  for(int s = 0; s + 1 < steps; ++s) {
+    issue(s + 1);
+    compute(s & 1);
+    commit((s + 1) & 1);
     __syncthreads();
   }
+  if(steps > 0) compute((steps - 1) & 1);
 
-#pragma unroll
-  for(int i = 0; i < FM; ++i)
-#pragma unroll
-    for(int j = 0; j < FN; ++j)
-      store_block32(p.ep, acc[i][j], tm * BM + wm * WM + i * 32, tn * BN + wn * WN + j * 32, lane);
+  static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
+    constexpr int i = decltype(ij)::value / FN, j = decltype(ij)::value % FN;
+    store_block32(p.ep, acc[i][j], tm * BM + wm * WM + i * 32, tn * BN + wn * WN + j * 32, lane);
+  });
 }
 
 struct BfConfig {
@@ -324,8 +330,8 @@ mfx_status launch_dense_bf16x(Handle* h, const DenseOp& op, bool split, int conf
     MFX_CHECK(pack(h, op.B, op.ldb, op.N, op.K, op.b_kmajor, pl.rows_pad_b, pl.kbs, nhl, pb, s));
   }
   BfParams p;
-  p.PA = pa;
-  p.PB = pb;
+  p.PA = reinterpret_cast<const u32x4*>(pa);
+  p.PB = reinterpret_cast<const u32x4*>(pb);
   p.kbs = (int)pl.kbs;
   p.nprod = op.nprod;
   p.tiles_m = (int)(pl.rows_pad_a / c.bm);

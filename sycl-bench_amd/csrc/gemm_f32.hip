@@ -139,7 +139,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f32_kernel(co
   const int KT = (op.K + BK - 1) / BK;
   const int steps = KT * op.nprod;
 
-  auto issue = [&](int s) {
+  auto issue = [&](int s) __attribute__((always_inline)) {
     const int prod = s / KT, kt = s - prod * KT;
     const float* pa = prod ? op.B : op.A;
     const float* pb = prod ? op.A : op.B;
@@ -148,18 +148,12 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f32_kernel(co
     la.load(pa, lda, op.M, op.K, m0, kt * BK, tid, va);
     lb.load(pb, ldb, op.N, op.K, n0, kt * BK, tid, vb);
   };
-
-  if(steps > 0) {
-    issue(0);
-    la.template store<Cfg::LDSA>(As[0], tid);
-    lb.template store<Cfg::LDSB>(Bs[0], tid);
-  }
-  __syncthreads();
-
+  auto commit = [&](int buf) __attribute__((always_inline)) {
+    la.template store<Cfg::LDSA>(As[buf], tid);
+    lb.template store<Cfg::LDSB>(Bs[buf], tid);
+  };
   const int lrow = lane & 31, lk = lane >> 5;
-  for(int s = 0; s < steps; ++s) {
-    const int cur = s & 1;
-    if(s + 1 < steps) issue(s + 1);
+  auto compute = [&](int cur) __attribute__((always_inline)) {
 #pragma unroll
     for(int kk = 0; kk < BK; kk += 2) {
       float a[FM], b[FN];
@@ -172,17 +166,27 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f32_kernel(co
 #pragma unroll
         for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
     }
-    if(s + 1 < steps) {
-      la.template store<Cfg::LDSA>(As[cur ^ 1], tid);
-      lb.template store<Cfg::LDSB>(Bs[cur ^ 1], tid);
-    }
+  };
+
+  // Steady state without branches around the prefetch, so the registers
+  // holding step s+1 stay live across the MFMAs of step s (no early wait).
+  if(steps > 0) {
+    issue(0);
+    commit(0);
+  }
+  __syncthreads();
+  for(int s = 0; s + 1 < steps; ++s) {
+    issue(s + 1);
+    compute(s & 1);
+    commit((s + 1) & 1);
     __syncthreads();
   }
+  if(steps > 0) compute((steps - 1) & 1);
 
-#pragma unroll
-  for(int i = 0; i < FM; ++i)
-#pragma unroll
-    for(int j = 0; j < FN; ++j) store_block32(op.ep, acc[i][j], m0 + wm * WM + i * 32, n0 + wn * WN + j * 32, lane);
+  static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
+    constexpr int i = decltype(ij)::value / FN, j = decltype(ij)::value % FN;
+    store_block32(op.ep, acc[i][j], m0 + wm * WM + i * 32, n0 + wn * WN + j * 32, lane);
+  });
 }
 
 struct F32Config {

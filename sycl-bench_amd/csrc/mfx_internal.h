@@ -9,6 +9,8 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/mfx.h"
@@ -188,14 +190,14 @@ __host__ __device__ inline float uniform_pm1(uint64_t h) {
 // Workgroups are dispatched round-robin over the 8 XCDs (wg i -> XCD i % 8).
 // This renumbers them so each XCD owns one contiguous range of tile ids, which
 // keeps the A/B panels a tile needs in that XCD's L2.
-__device__ inline int xcd_remap(int bid, int nblocks) {
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
   const int xcd = bid & 7, local = bid >> 3;
   const int per = nblocks >> 3, rem = nblocks & 7;
   return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
 
 // Grouped ordering: GROUP consecutive tile rows are walked column by column.
-__device__ inline void grouped_tile(int pid, int tiles_m, int tiles_n, int group, int& tm, int& tn) {
+__device__ __forceinline__ void grouped_tile(int pid, int tiles_m, int tiles_n, int group, int& tm, int& tn) {
   const int in_group = group * tiles_n;
   const int gid = pid / in_group;
   const int first_m = gid * group;
@@ -206,7 +208,7 @@ __device__ inline void grouped_tile(int pid, int tiles_m, int tiles_n, int group
 }
 
 // Tile (ti, tj) with tj <= ti for linear index p over the lower triangle.
-__device__ inline void lower_tile(int p, int& ti, int& tj) {
+__device__ __forceinline__ void lower_tile(int p, int& ti, int& tj) {
   int i = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
   while(i * (i + 1) / 2 > p) --i;
   while((i + 1) * (i + 2) / 2 <= p) ++i;
@@ -230,12 +232,42 @@ struct Epilogue {
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [0, N).  Used
+// where an index selects an accumulator block, so the index stays a constant
+// and the accumulators stay in registers (a runtime index, or a loop the
+// unroller gives up on, moves the whole array to scratch memory).
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
 // Writes one 32x32 MFMA accumulator block whose top-left corner is (r0, c0).
 // v_mfma_f32_32x32x* layout: lane l, register q holds
 // row 8*(q/4) + 4*(l/32) + q%4, column l%32.
-__device__ inline void store_block32(const Epilogue& ep, const floatx16& acc, int r0, int c0, int lane) {
+// Always inlined and taking the block by value: a call would force the whole
+// accumulator array into scratch memory.
+__device__ __forceinline__ void store_block32(const Epilogue& ep, floatx16 acc, int r0, int c0, int lane) {
   const int col = c0 + (lane & 31);
   const int rbase = r0 + 4 * (lane >> 5);
+  if(ep.uplo == TILES_FULL && r0 + 32 <= ep.M && c0 + 32 <= ep.N) {
+    // Interior block: each register q is one 32-float row segment per half wave.
+    float* c = ep.C + (int64_t)rbase * ep.ldc + col;
+    if(ep.beta != 0.0f) {
+      float old[16];
+#pragma unroll
+      for(int q = 0; q < 16; ++q) old[q] = c[(int64_t)(8 * (q >> 2) + (q & 3)) * ep.ldc];
+#pragma unroll
+      for(int q = 0; q < 16; ++q) c[(int64_t)(8 * (q >> 2) + (q & 3)) * ep.ldc] = ep.alpha * acc[q] + ep.beta * old[q];
+    } else {
+#pragma unroll
+      for(int q = 0; q < 16; ++q) c[(int64_t)(8 * (q >> 2) + (q & 3)) * ep.ldc] = ep.alpha * acc[q];
+    }
+    return;
+  }
 #pragma unroll
   for(int q = 0; q < 16; ++q) {
     const int row = rbase + 8 * (q >> 2) + (q & 3);
