@@ -33,6 +33,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(n, kernel):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE
+    summary (profiles/gemm_n8192_traffic.json, counter-calibrated); None if absent."""
+    path = os.path.join(ROOT, "profiles", f"gemm_n{n}_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"][kernel]["bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def time_steps(torch, h, A, B, C, prec, steps, warmup, alpha, beta):
     for _ in range(warmup):
         h.gemm(A, B, C, alpha=alpha, beta=beta, prec=prec)
@@ -91,6 +102,9 @@ def main():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="short pass for counter collection: one split-BF16 and one FP32 gemm at --n, plus a "
+                         "1 GiB device copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the dense-contraction path is built and measured on a single "
@@ -110,6 +124,20 @@ def main():
     h.fill_uniform(C0, 1, 2)
     alpha, beta = 1.5, 1.2
     h.reserve_workspace(mfx.load().mfx_gemm_workspace_size(mfx.SPLIT_BF16, n, n, n))
+
+    if args.profile_pass:
+        C = C0.clone()
+        for prec in ("split-bf16", "fp32"):
+            h.gemm(A, B, C, alpha=alpha, beta=beta, prec=prec)
+            torch.cuda.synchronize()
+            log(f"profile pass: {prec} gemm N={n} done")
+        src = torch.empty(1 << 28, device=dev)  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("profile pass: 1 GiB device copy done (calibration: 1048576 KiB read, 1048576 KiB written)")
+        h.close()
+        return
 
     # accuracy: one call from the seeded C against the GPU FP64 oracle
     log("FP64 oracle ...")
@@ -170,8 +198,9 @@ def main():
         "split_fp32_equiv_frac": round(value / PEAK_FP32, 3),
         "pack_ms_per_step": round(pack_ms, 4),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_BF16, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_BF16, 4), "traffic": None,
-                     "kernel": "gemm_split_bf16_mfma", "kernel_ms": round(k_avg, 4)},
+                     "frac": round(achieved / PEAK_BF16, 4), "traffic": measured_traffic(n, "gemm_split_bf16_mfma"),
+                     "kernel": "gemm_split_bf16_mfma", "kernel_ms": round(k_avg, 4),
+                     "achieved_definition": "3 BF16 MFMA products x 2N^3 flop / average kernel duration"},
     }
     if not args.no_cpu:
         log("CPU baseline ...")

@@ -235,6 +235,8 @@ mfx_status mfx_create(mfx_handle* handle, int device) {
   hs->h.device = device;
   hs->h.cu_count = prop.multiProcessorCount;
   hs->h.device_name = prop.name;
+  // The marketing name can be empty when the driver's id table is missing.
+  if(hs->h.device_name.empty()) hs->h.device_name = std::string("AMD GPU ") + prop.gcnArchName;
   *handle = hs;
   return MFX_SUCCESS;
 }
@@ -242,16 +244,20 @@ mfx_status mfx_create(mfx_handle* handle, int device) {
 mfx_status mfx_destroy(mfx_handle handle) {
   if(!handle) return MFX_SUCCESS;
   Handle* h = &handle->h;
-  hipSetDevice(h->device);
-  hipDeviceSynchronize();
+  // Teardown continues past errors; the first one is reported.
+  hipError_t first = hipSetDevice(h->device);
+  auto keep = [&first](hipError_t e) {
+    if(first == hipSuccess) first = e;
+  };
+  keep(hipDeviceSynchronize());
   for(auto& r : h->records) {
-    hipEventDestroy(r.start);
-    hipEventDestroy(r.stop);
+    keep(hipEventDestroy(r.start));
+    keep(hipEventDestroy(r.stop));
   }
-  for(auto e : h->free_events) hipEventDestroy(e);
-  if(h->workspace) hipFree(h->workspace);
+  for(auto e : h->free_events) keep(hipEventDestroy(e));
+  if(h->workspace) keep(hipFree(h->workspace));
   delete handle;
-  return MFX_SUCCESS;
+  return first == hipSuccess ? MFX_SUCCESS : hip_fail(first, "mfx_destroy");
 }
 
 mfx_status mfx_reserve_workspace(mfx_handle handle, size_t bytes) {
