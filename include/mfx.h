@@ -141,13 +141,22 @@ mfx_status mfx_gemm_grouped(mfx_handle handle, mfx_precision prec,
 mfx_status mfx_chain(mfx_handle handle, mfx_precision prec, int32_t count,
     const float* const* mats, int64_t n, float* const* scratch, float* out, mfx_stream stream);
 
+/* Blocking: waits for `stream`, then reports MFX_ERROR_TIMEOUT if an in-kernel
+ * dependency wait of the handle's last mfx_gemm_grouped / mfx_chain launch
+ * exceeded its spin limit (the launch then skipped work; outputs are invalid). */
+mfx_status mfx_grouped_status(mfx_handle handle, mfx_stream stream);
+
 /* Graph form of mfx_gemm_grouped: captures the grouped launch into a hipGraph
- * once (instantiate) and replays it (launch).  The problem list is copied. */
+ * once (instantiate) and replays it (launch).  The problem list is copied; the
+ * graph owns its work tables and packed panels (device memory freed by
+ * mfx_graph_destroy).  Replays of one graph must not overlap. */
 typedef struct mfx_graph_s* mfx_graph;
 mfx_status mfx_grouped_graph_create(mfx_handle handle, mfx_precision prec,
     const mfx_gemm_problem* problems, int32_t count, const mfx_dep* deps, int32_t ndeps,
     mfx_stream stream, mfx_graph* graph);
 mfx_status mfx_graph_launch(mfx_graph graph, mfx_stream stream);
+/* Blocking: like mfx_grouped_status for the graph's last replay. */
+mfx_status mfx_graph_status(mfx_graph graph, mfx_stream stream);
 mfx_status mfx_graph_destroy(mfx_graph graph);
 
 /* ---- FP64 oracle and error metrics (checkers, not part of any product) ---- */
@@ -218,6 +227,16 @@ mfx_status mfx_timing_end(mfx_handle handle, int32_t cap, const char** names,
  */
 mfx_status mfx_autotune(mfx_handle handle, mfx_precision prec, int64_t m, int64_t n,
     int64_t k, int32_t* best_config, mfx_stream stream);
+/*
+ * Tunes the per-problem tile choice of a grouped launch on the caller's own
+ * problem list: times the launch under several padding-waste bounds, keeps the
+ * fastest for the handle and records each problem's resulting tile in the
+ * table.  Runs the launch several times on the given buffers (C is
+ * overwritten).  Blocking.  *best_waste receives the chosen bound.
+ */
+mfx_status mfx_autotune_grouped(mfx_handle handle, mfx_precision prec,
+    const mfx_gemm_problem* problems, int32_t count, const mfx_dep* deps, int32_t ndeps,
+    double* best_waste, mfx_stream stream);
 /* Saves / loads the tuning table as JSON. */
 mfx_status mfx_autotune_save(const char* path);
 mfx_status mfx_autotune_load(const char* path);

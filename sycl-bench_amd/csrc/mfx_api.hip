@@ -172,6 +172,26 @@ int choose_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k) {
   return heuristic_config(h, p, m, n, k);
 }
 
+// Grouped launches fill the machine with many problems at once, so each
+// problem takes the largest tile whose padded area exceeds the problem's by at
+// most `grouped_waste` (tuned entries under the "g" prefix take precedence).
+int choose_grouped_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k) {
+  if(h->forced_config[p] >= 0) return h->forced_config[p];
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    auto it = g_tuned.find("g" + tune_key(p, m, n, k));
+    if(it != g_tuned.end()) return it->second;
+  }
+  const int order[4] = {0, 3, 1, 2};
+  for(int oi = 0; oi < 4; ++oi) {
+    int bm, bn, bk, w;
+    if(!config_info(p, order[oi], &bm, &bn, &bk, &w)) continue;
+    const double padded = (double)round_up(m, bm) * (double)round_up(n, bn);
+    if(padded <= h->grouped_waste * (double)m * (double)n) return order[oi];
+  }
+  return 2;
+}
+
 // Square-tile choice for the triangular schedules.
 int choose_square_config(Handle* h, mfx_precision p, int64_t n, int64_t k) {
   int c = choose_config(h, p, n, n, k);
@@ -234,6 +254,7 @@ mfx_status mfx_create(mfx_handle* handle, int device) {
   if(!hs) return MFX_ERROR_ALLOC;
   hs->h.device = device;
   hs->h.cu_count = prop.multiProcessorCount;
+  if(const char* sl = std::getenv("MFX_SPIN_LIMIT")) hs->h.spin_limit = std::max(0, std::atoi(sl));
   hs->h.device_name = prop.name;
   // The marketing name can be empty when the driver's id table is missing.
   if(hs->h.device_name.empty()) hs->h.device_name = std::string("AMD GPU ") + prop.gcnArchName;
@@ -256,6 +277,9 @@ mfx_status mfx_destroy(mfx_handle handle) {
   }
   for(auto e : h->free_events) keep(hipEventDestroy(e));
   if(h->workspace) keep(hipFree(h->workspace));
+  if(h->staging) keep(hipHostFree(h->staging));
+  if(h->staging_ready) keep(hipEventDestroy(h->staging_ready));
+  if(h->error_flag) keep(hipFree(h->error_flag));
   delete handle;
   return first == hipSuccess ? MFX_SUCCESS : hip_fail(first, "mfx_destroy");
 }

@@ -58,6 +58,15 @@ struct Handle {
   std::vector<hipEvent_t> free_events;
   int forced_config[5] = {-1, -1, -1, -1, -1};
   bool capturing = false;  // set while a grouped launch is being captured
+  // Grouped launches: spin limit of the in-kernel dependency waits (env
+  // MFX_SPIN_LIMIT overrides), pinned staging for the work tables and the
+  // event that marks their upload, and the error flag of the last launch.
+  int spin_limit = 1 << 22;
+  double grouped_waste = 1.34;  // padded-area bound of the grouped tile choice
+  void* staging = nullptr;
+  size_t staging_bytes = 0;
+  hipEvent_t staging_ready = nullptr;
+  int* error_flag = nullptr;
 };
 
 // Returns a workspace pointer of at least `bytes` (grows the arena; growing

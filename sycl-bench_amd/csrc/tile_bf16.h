@@ -1,0 +1,224 @@
+// Split-BF16 / single-BF16 building blocks shared by the single-problem
+// kernels (gemm_bf16.hip) and the persistent grouped kernel (grouped.hip).
+//
+// Split BF16: every FP32 operand x is written as hi + lo with
+// hi = bf16_rne(x), lo = bf16_rne(x - hi); the product uses
+// lo*hi + hi*lo + hi*hi (the lo*lo term, ~2^-16 relative, is dropped) with
+// FP32 accumulation.  Single BF16 keeps only hi*hi.
+//
+// Packed "fragment order" layout of an operand (rows x K, rows = M for the A
+// role, N for the B role):
+//     P[blk][kb][hl][lane][8]   (bf16; one 16-byte chunk per lane)
+// blk = 32-row block, kb = 16-wide k block, hl = hi / lo, and lane l holds
+// row blk*32 + l%32, k = kb*16 + 8*(l/32) + 0..7 -- the register operand of
+// v_mfma_f32_32x32x16_bf16 for both the A and the B role.  Rows are padded to
+// the tile size and k to the 32-wide stage with zeros.
+#pragma once
+
+#include "mfx_internal.h"
+
+namespace mfx {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+// Native clang vector (not HIP's uint4 struct) so register arrays of chunks are
+// promoted to registers instead of living in scratch memory.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int KB_PER_STAGE = 2;  // BK = 32
+
+__device__ __forceinline__ uint32_t pack_bf16_pair(uint16_t a, uint16_t b) {
+  return (uint32_t)a | ((uint32_t)b << 16);
+}
+
+// Source view of one operand for packing.
+struct PackSrc {
+  const float* src;
+  int64_t ld;
+  int rows, K;  // logical extent (rows x K)
+  int kmajor;   // element (r, k) at src[r*ld + k], else src[k*ld + r]
+  int vec;      // 128-bit loads allowed (kmajor, aligned rows)
+};
+
+// Host-side constructor (decides whether 128-bit loads are allowed).
+PackSrc make_pack_src(const float* src, int64_t ld, int rows, int K, bool kmajor);
+
+// Packs lane-chunk (blk, kb, lane): 8 consecutive k of one row, hi (and lo).
+// `dst` points at the packed operand; kbs = number of 16-wide k blocks.
+template <int NHL>
+__device__ __forceinline__ void pack_chunk(const PackSrc& s, int blk, int kb, int lane, int kbs, u32x4* dst) {
+  const int r = blk * 32 + (lane & 31);
+  const int k0 = kb * 16 + (lane >> 5) * 8;
+  float v[8];
+  if(s.kmajor) {
+    const float* p = s.src + (int64_t)r * s.ld + k0;
+    if(r < s.rows && s.vec && k0 + 8 <= s.K) {
+      const float4 x = reinterpret_cast<const float4*>(p)[0];
+      const float4 y = reinterpret_cast<const float4*>(p)[1];
+      v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+      v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
+    } else {
+#pragma unroll
+      for(int e = 0; e < 8; ++e) v[e] = (r < s.rows && k0 + e < s.K) ? p[e] : 0.0f;
+    }
+  } else {
+#pragma unroll
+    for(int e = 0; e < 8; ++e) v[e] = (r < s.rows && k0 + e < s.K) ? s.src[(int64_t)(k0 + e) * s.ld + r] : 0.0f;
+  }
+  uint16_t hi[8], lo[8];
+#pragma unroll
+  for(int e = 0; e < 8; ++e) {
+    hi[e] = bf16_rne(v[e]);
+    lo[e] = bf16_rne(v[e] - bf16_to_f32(hi[e]));
+  }
+  const int64_t base = ((int64_t)blk * kbs + kb) * NHL * 64 + lane;
+  u32x4 h;
+  h.x = pack_bf16_pair(hi[0], hi[1]);
+  h.y = pack_bf16_pair(hi[2], hi[3]);
+  h.z = pack_bf16_pair(hi[4], hi[5]);
+  h.w = pack_bf16_pair(hi[6], hi[7]);
+  dst[base] = h;
+  if(NHL == 2) {
+    u32x4 l;
+    l.x = pack_bf16_pair(lo[0], lo[1]);
+    l.y = pack_bf16_pair(lo[2], lo[3]);
+    l.z = pack_bf16_pair(lo[4], lo[5]);
+    l.w = pack_bf16_pair(lo[6], lo[7]);
+    dst[base + 64] = l;
+  }
+}
+
+template <int BM, int BN, int WM, int WN, int NHL>
+struct BfCfg {
+  static constexpr int NWM = BM / WM, NWN = BN / WN, NW = NWM * NWN, NT = NW * 64;
+  static constexpr int FM = WM / 32, FN = WN / 32;
+  static constexpr int CH_A = BM / 32 * KB_PER_STAGE * NHL * 64;  // 16-byte chunks per stage
+  static constexpr int CH_B = BN / 32 * KB_PER_STAGE * NHL * 64;
+  static constexpr int PER_A = CH_A / NT, PER_B = CH_B / NT;
+  static constexpr int SMEM_CHUNKS = 2 * (CH_A + CH_B);
+  static_assert(CH_A % NT == 0 && CH_B % NT == 0, "stage must split evenly");
+};
+
+// Packed operands of one contraction.
+struct BfTileArgs {
+  const u32x4* PA;  // packed A-role operand of product 0
+  const u32x4* PB;  // packed B-role operand of product 0
+  int kbs;          // k blocks of 16 in both packed operands
+  int nprod;        // 2: second product uses (PB, PA)
+  Epilogue ep;
+};
+
+__device__ __forceinline__ bf16x8 as_bf16x8(u32x4 u) { return __builtin_bit_cast(bf16x8, u); }
+
+// Computes tile (tm, tn) and writes it through t.ep.  `lds` holds
+// BfCfg::SMEM_CHUNKS chunks; the caller separates two calls that reuse it
+// with a barrier.
+template <int BM, int BN, int WM, int WN, int NHL>
+__device__ __forceinline__ void bf16_tile(const BfTileArgs& t, int tm, int tn, u32x4* lds) {
+  using Cfg = BfCfg<BM, BN, WM, WN, NHL>;
+  constexpr int NT = Cfg::NT, FM = Cfg::FM, FN = Cfg::FN;
+  constexpr int SA = Cfg::CH_A, SB = Cfg::CH_B;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm = wid / Cfg::NWN, wn = wid % Cfg::NWN;
+
+  floatx16 acc[FM][FN];
+#pragma unroll
+  for(int i = 0; i < FM; ++i)
+#pragma unroll
+    for(int j = 0; j < FN; ++j)
+#pragma unroll
+      for(int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
+
+  // Chunk c of a stage: block c / RUN of the tile, then the contiguous
+  // (kb, hl, lane) run of that block for this stage.
+  constexpr int RUN = KB_PER_STAGE * NHL * 64;
+  const int KT = t.kbs / KB_PER_STAGE;
+  const int steps = KT * t.nprod;
+  const int64_t blk_stride = (int64_t)t.kbs * NHL * 64;
+  u32x4 ra[Cfg::PER_A], rb[Cfg::PER_B];
+
+  auto issue = [&](int s) __attribute__((always_inline)) {
+    const int prod = s / KT, kt = s - prod * KT;
+    const u32x4* A = prod ? t.PB : t.PA;
+    const u32x4* B = prod ? t.PA : t.PB;
+    const int64_t koff = (int64_t)kt * RUN;
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_A; ++i) {
+      const int c = tid + i * NT;
+      const int blk = c / RUN, run = c - blk * RUN;
+      ra[i] = A[(int64_t)(tm * (BM / 32) + blk) * blk_stride + koff + run];
+    }
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_B; ++i) {
+      const int c = tid + i * NT;
+      const int blk = c / RUN, run = c - blk * RUN;
+      rb[i] = B[(int64_t)(tn * (BN / 32) + blk) * blk_stride + koff + run];
+    }
+  };
+  auto commit = [&](int buf) __attribute__((always_inline)) {
+    u32x4* sa = lds + buf * (SA + SB);
+    u32x4* sb = sa + SA;
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_A; ++i) sa[tid + i * NT] = ra[i];
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_B; ++i) sb[tid + i * NT] = rb[i];
+  };
+  auto compute = [&](int cur) __attribute__((always_inline)) {
+    const u32x4* sa = lds + cur * (SA + SB);
+    const u32x4* sb = sa + SA;
+#pragma unroll
+    for(int kb = 0; kb < KB_PER_STAGE; ++kb) {
+      bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+      for(int i = 0; i < FM; ++i) {
+        const int blk = wm * FM + i;
+        ah[i] = as_bf16x8(sa[(blk * KB_PER_STAGE + kb) * NHL * 64 + lane]);
+        if(NHL == 2) al[i] = as_bf16x8(sa[((blk * KB_PER_STAGE + kb) * NHL + 1) * 64 + lane]);
+      }
+#pragma unroll
+      for(int j = 0; j < FN; ++j) {
+        const int blk = wn * FN + j;
+        bh[j] = as_bf16x8(sb[(blk * KB_PER_STAGE + kb) * NHL * 64 + lane]);
+        if(NHL == 2) bl[j] = as_bf16x8(sb[((blk * KB_PER_STAGE + kb) * NHL + 1) * 64 + lane]);
+      }
+      // Product-major order: consecutive MFMAs write different accumulators,
+      // so none waits on the previous one's result.  The small cross terms go
+      // in first so they are added while the accumulator is smallest.
+      if(NHL == 2) {
+#pragma unroll
+        for(int i = 0; i < FM; ++i)
+#pragma unroll
+          for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for(int i = 0; i < FM; ++i)
+#pragma unroll
+          for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+      }
+#pragma unroll
+      for(int i = 0; i < FM; ++i)
+#pragma unroll
+        for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+    }
+  };
+
+  // Steady state without branches around the prefetch, so the registers
+  // holding stage s+1 stay live across the MFMAs of stage s (no early wait).
+  if(steps > 0) {
+    issue(0);
+    commit(0);
+  }
+  __syncthreads();
+  for(int s = 0; s + 1 < steps; ++s) {
+    issue(s + 1);
+    compute(s & 1);
+    commit((s + 1) & 1);
+    __syncthreads();
+  }
+  if(steps > 0) compute((steps - 1) & 1);
+
+  static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
+    constexpr int i = decltype(ij)::value / FN, j = decltype(ij)::value % FN;
+    store_block32(t.ep, acc[i][j], tm * BM + wm * WM + i * 32, tn * BN + wn * WN + j * 32, lane);
+  });
+}
+
+}  // namespace mfx

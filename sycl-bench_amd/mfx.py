@@ -57,7 +57,11 @@ SIGNATURES = {
     "mfx_grouped_graph_create": (_ST, [_V, _ST, ctypes.POINTER(Problem), _I32, ctypes.POINTER(Dep), _I32, _V,
                                        ctypes.POINTER(_V)]),
     "mfx_graph_launch": (_ST, [_V, _V]),
+    "mfx_graph_status": (_ST, [_V, _V]),
     "mfx_graph_destroy": (_ST, [_V]),
+    "mfx_grouped_status": (_ST, [_V, _V]),
+    "mfx_autotune_grouped": (_ST, [_V, _ST, ctypes.POINTER(Problem), _I32, ctypes.POINTER(Dep), _I32,
+                                   ctypes.POINTER(_D), _V]),
     "mfx_gemm_f64": (_ST, [_V, _ST, _ST, _I64, _I64, _I64, _D, _V, _I64, _V, _I64, _D, _V, _I64, _V]),
     "mfx_convert_f32_f64": (_ST, [_V, _I64, _I64, _V, _I64, _V, _I64, _V]),
     "mfx_rel_frobenius": (_ST, [_V, _ST, _I64, _I64, _V, _I64, _V, _I64, ctypes.POINTER(_D), _V]),
@@ -233,6 +237,17 @@ class Handle:
         _check(self.lib.mfx_gemm_grouped(self.h, prec_id(prec), arr, len(problems), darr, len(deps),
                                          _stream(stream)), "mfx_gemm_grouped")
 
+    def grouped_status(self, stream=None):
+        """Blocking; raises MfxError(MFX_ERROR_TIMEOUT) if the last grouped launch skipped work."""
+        _check(self.lib.mfx_grouped_status(self.h, _stream(stream)), "mfx_grouped_status")
+
+    def autotune_grouped(self, problems, deps=(), prec="split-bf16", stream=None):
+        arr, darr = _problem_arrays(problems, deps)
+        w = ctypes.c_double()
+        _check(self.lib.mfx_autotune_grouped(self.h, prec_id(prec), arr, len(problems), darr, len(deps),
+                                             ctypes.byref(w), _stream(stream)), "mfx_autotune_grouped")
+        return w.value
+
     def chain(self, mats, scratch, out, prec="split-bf16", stream=None):
         n = out.shape[0]
         mp = (ctypes.c_void_p * len(mats))(*[m.data_ptr() for m in mats])
@@ -260,6 +275,9 @@ class Graph:
 
     def launch(self, stream=None):
         _check(self.lib.mfx_graph_launch(self.g, _stream(stream)), "mfx_graph_launch")
+
+    def status(self, stream=None):
+        _check(self.lib.mfx_graph_status(self.g, _stream(stream)), "mfx_graph_status")
 
     def close(self):
         if self.g:
