@@ -62,6 +62,95 @@ def time_steps(torch, h, A, B, C, prec, steps, warmup, alpha, beta):
     return start.elapsed_time(stop), wall, kernels
 
 
+def grouped_workload(torch, mfx, h, steps, warmup, seed=1):
+    """512 independent problems (M, N, K log-uniform on [64, 2048], multiples of
+    16, from `seed`) plus an 8-long chain of 2048 x 2048 matrices, as one
+    persistent grouped launch in a hipGraph with autotuned tiles, split-BF16."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    dev = "cuda"
+    probs, mats, flop = [], [], 0.0
+    dims = np.clip(np.rint(np.exp(rng.uniform(np.log(64), np.log(2048), (512, 3))) / 16) * 16, 64, 2048).astype(int)
+    for i, (m, n, k) in enumerate(dims):
+        A = torch.empty((m, k), device=dev)
+        B = torch.empty((k, n), device=dev)
+        C = torch.empty((m, n), device=dev)
+        h.fill_uniform(A, seed, 3 * i)
+        h.fill_uniform(B, seed, 3 * i + 1)
+        h.fill_uniform(C, seed, 3 * i + 2)
+        mats.append((A, B, C))
+        probs.append(mfx.problem(A, B, C, alpha=1.0, beta=0.0))
+        flop += 2.0 * m * n * k
+    n = 2048
+    chain = [torch.empty((n, n), device=dev) for _ in range(8)]
+    for j, x in enumerate(chain):
+        h.fill_uniform(x, seed, 10000 + j)
+    inter = [torch.empty((n, n), device=dev) for _ in range(7)]
+    # ((M0 M1)(M2 M3))((M4 M5)(M6 M7)): inter[0..3] level 1, inter[4..5] level 2, inter[6] the product
+    base = len(probs)
+    pairs = [(chain[0], chain[1]), (chain[2], chain[3]), (chain[4], chain[5]), (chain[6], chain[7]),
+             (inter[0], inter[1]), (inter[2], inter[3]), (inter[4], inter[5])]
+    deps = [(base + 4, base + 0), (base + 4, base + 1), (base + 5, base + 2), (base + 5, base + 3),
+            (base + 6, base + 4), (base + 6, base + 5)]
+    for (x, y), out in zip(pairs, inter):
+        probs.append(mfx.problem(x, y, out))
+        flop += 2.0 * n ** 3
+    torch.cuda.synchronize()
+
+    log("grouped: autotuning tile policy ...")
+    policy = h.autotune_grouped(probs, deps, prec="split-bf16")
+    big, small, bound = mfx.grouped_policy_info(policy)
+    graph = h.graph_create(probs, deps, prec="split-bf16")
+    for _ in range(warmup):
+        graph.launch()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        graph.launch()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    graph.status()
+    ms = e0.elapsed_time(e1) / steps
+    # accuracy of the chain product and of a sample of the independent problems
+    ref = chain[0].double()
+    level = [c.double() for c in chain]
+    while len(level) > 1:
+        level = [level[i] @ level[i + 1] for i in range(0, len(level), 2)]
+    chain_err = (torch.linalg.norm(inter[6].double() - level[0]) / torch.linalg.norm(level[0])).item()
+    worst = 0.0
+    for A, B, C in mats[::32]:
+        r = A.double() @ B.double()
+        worst = max(worst, (torch.linalg.norm(C.double() - r) / torch.linalg.norm(r)).item())
+    del ref, level
+    # the same work as one mfx_gemm launch per problem
+    t_single = 0.0
+    for i, p in enumerate(probs[:base]):
+        A, B, C = mats[i]
+        h.gemm(A, B, C, prec="split-bf16")
+    torch.cuda.synchronize()
+    s0 = torch.cuda.Event(enable_timing=True)
+    s1 = torch.cuda.Event(enable_timing=True)
+    s0.record(stream)
+    for i in range(base):
+        A, B, C = mats[i]
+        h.gemm(A, B, C, prec="split-bf16")
+    for (x, y), out in zip(pairs, inter):
+        h.gemm(x, y, out, prec="split-bf16")
+    s1.record(stream)
+    torch.cuda.synchronize()
+    t_single = s0.elapsed_time(s1)
+    graph.close()
+    return {"workload": "512 uneven problems (M,N,K log-uniform [64,2048], x16, seed 1) + 8-long chain of "
+                        "2048^2, split-BF16, one persistent grouped launch replayed from a hipGraph",
+            "flop": flop, "ms": round(ms, 4), "tflops": round(flop / (ms / 1e3) / 1e12, 2),
+            "per_problem_launches_ms": round(t_single, 4),
+            "per_problem_launches_tflops": round(flop / (t_single / 1e3) / 1e12, 2),
+            "tile_policy": {"id": policy, "large_config": big, "small_config": small, "waste_bound": bound},
+            "chain_rel_err": chain_err, "sampled_rel_err_max": worst}
+
+
 def cpu_baseline(n, target_s):
     """OpenMP restatement of polybench/gemm.cpp:82-91 on the host cores, timed
     on rows [0, R) of the same N x N problem (R sized to ~target_s)."""
@@ -102,6 +191,7 @@ def main():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-grouped", action="store_true")
     ap.add_argument("--profile-pass", action="store_true",
                     help="short pass for counter collection: one split-BF16 and one FP32 gemm at --n, plus a "
                          "1 GiB device copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters")
@@ -202,6 +292,11 @@ def main():
                      "kernel": "gemm_split_bf16_mfma", "kernel_ms": round(k_avg, 4),
                      "achieved_definition": "3 BF16 MFMA products x 2N^3 flop / average kernel duration"},
     }
+    if not args.no_grouped:
+        g = grouped_workload(torch, mfx, h, max(3, args.steps), max(1, args.warmup))
+        g["frac_of_large_tile"] = round(g["tflops"] / value, 3)
+        out["grouped"] = g
+        log(f"grouped: {g}")
     if not args.no_cpu:
         log("CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)

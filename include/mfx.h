@@ -228,15 +228,28 @@ mfx_status mfx_timing_end(mfx_handle handle, int32_t cap, const char** names,
 mfx_status mfx_autotune(mfx_handle handle, mfx_precision prec, int64_t m, int64_t n,
     int64_t k, int32_t* best_config, mfx_stream stream);
 /*
- * Tunes the per-problem tile choice of a grouped launch on the caller's own
- * problem list: times the launch under several padding-waste bounds, keeps the
- * fastest for the handle and records each problem's resulting tile in the
- * table.  Runs the launch several times on the given buffers (C is
- * overwritten).  Blocking.  *best_waste receives the chosen bound.
+ * Tunes the tile policy of a grouped launch on the caller's own problem list:
+ * times the launch under every policy (see mfx_grouped_policy_info), records
+ * the fastest for launches with the same key (precision, problem count,
+ * contraction volume and dependency depth, in power-of-two buckets) and
+ * returns its id.  Runs the launch several times on the given buffers (C is
+ * overwritten).  Blocking.
  */
 mfx_status mfx_autotune_grouped(mfx_handle handle, mfx_precision prec,
     const mfx_gemm_problem* problems, int32_t count, const mfx_dep* deps, int32_t ndeps,
-    double* best_waste, mfx_stream stream);
+    int32_t* best_policy, mfx_stream stream);
+/*
+ * Grouped tile policies: a launch runs one policy, a pair of tile
+ * configurations (ids as in mfx_config_info); each problem takes the large
+ * one unless that pads its output area by more than `waste_bound` (0: always
+ * the large one), else the small one.  FP32 launches run configuration 3 in
+ * place of 0.
+ */
+int32_t mfx_grouped_policy_count(void);
+mfx_status mfx_grouped_policy_info(int32_t policy, int32_t* large_config, int32_t* small_config,
+    double* waste_bound);
+/* Forces `policy` (or -1 for the tuned / estimated choice) for later grouped launches. */
+mfx_status mfx_force_grouped_policy(mfx_handle handle, int32_t policy);
 /* Saves / loads the tuning table as JSON. */
 mfx_status mfx_autotune_save(const char* path);
 mfx_status mfx_autotune_load(const char* path);

@@ -1,7 +1,7 @@
 // The autotuner: times every compiled tile configuration of a precision on a
 // problem shape and records the fastest per (precision, shape bucket); for
-// grouped launches it times tile-choice policies on the caller's own problem
-// list.  Choices live in a process-wide table (mutex-protected) that
+// grouped launches it times every tile policy on the caller's own problem
+// list and records the fastest for that launch's key.  Choices live in a process-wide table (mutex-protected) that
 // mfx_gemm / mfx_syrk / mfx_syr2k / grouped launches consult, and can be
 // saved to and loaded from a JSON file.
 #include <cstdio>
@@ -11,6 +11,7 @@
 #include <map>
 #include <sstream>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "mfx_internal.h"
@@ -22,7 +23,10 @@ void record_tuned(const std::string& key, int config);
 std::map<std::string, int> tuned_snapshot();
 void replace_tuned(const std::map<std::string, int>& t);
 mfx_status dispatch_dense(Handle* h, mfx_precision prec, DenseOp& op, int config, hipStream_t s);
-int choose_grouped_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k);
+int grouped_policy_count();
+bool grouped_policy_info(int id, int* big, int* small, double* bound);
+std::string grouped_tune_key(mfx_precision p, const mfx_gemm_problem* problems, int count, const mfx_dep* deps,
+    int ndeps);
 mfx_status run_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* problems, int count,
     const mfx_dep* deps, int ndeps, hipStream_t s);
 mfx_status launch_fill_uniform(Handle* h, int64_t rows, int64_t cols, float* p, int64_t ld, uint64_t seed,
@@ -69,8 +73,6 @@ std::string json_escape(const std::string& s) {
   }
   return o;
 }
-
-double g_best_waste = 0.0;  // grouped policy chosen by the last grouped tuning (0: none)
 
 }  // namespace
 
@@ -135,45 +137,70 @@ mfx_status mfx_autotune(mfx_handle handle, mfx_precision prec, int64_t m, int64_
 }
 
 mfx_status mfx_autotune_grouped(mfx_handle handle, mfx_precision prec, const mfx_gemm_problem* problems,
-    int32_t count, const mfx_dep* deps, int32_t ndeps, double* best_waste, mfx_stream stream) {
+    int32_t count, const mfx_dep* deps, int32_t ndeps, int32_t* best_policy, mfx_stream stream) {
   if(!handle) return invalid("null handle");
   Handle* h = &handle->h;
   MFX_HIP(hipSetDevice(h->device));
-  hipStream_t s = static_cast<hipStream_t>(stream);
+  if(prec != MFX_PREC_FP32 && prec != MFX_PREC_SPLIT_BF16 && prec != MFX_PREC_BF16)
+    return unsupported("grouped launch supports FP32, SPLIT_BF16 and BF16");
   if(h->forced_config[prec] >= 0) return invalid("a forced configuration overrides grouped tuning; clear it first");
-  // Candidate policies: the padded-area bound of the per-problem tile choice.
-  const double candidates[] = {1.0, 1.15, 1.34, 1.6, 2.0, 3.0, 1e9};
-  const double saved = h->grouped_waste;
+  hipStream_t s = static_cast<hipStream_t>(stream);
   const bool timing = h->timing;
+  const int forced_policy = h->grouped_policy;
   h->timing = false;
-  // Drop earlier grouped entries so the policies are what is measured.
-  auto table = tuned_snapshot();
-  for(auto it = table.begin(); it != table.end();) it = it->first[0] == 'g' ? table.erase(it) : std::next(it);
-  replace_tuned(table);
-  double best = saved, best_ms = -1.0;
+  int best = -1;
+  double best_ms = 0.0;
   mfx_status st = MFX_SUCCESS;
-  for(double w : candidates) {
-    h->grouped_waste = w;
+  for(int p = 0; p < grouped_policy_count() && st == MFX_SUCCESS; ++p) {
+    // FP32 runs 256x128 for the 256x256 configuration: time each distinct
+    // pair of configurations once.
+    auto pair_of = [&](int q) {
+      int big, small;
+      double bound;
+      grouped_policy_info(q, &big, &small, &bound);
+      if(prec == MFX_PREC_FP32) {
+        big = big == 0 ? 3 : big;
+        small = small == 0 ? 3 : small;
+      }
+      return std::make_tuple(big, small, big == small ? 0.0 : bound);
+    };
+    bool dup = false;
+    for(int q = 0; q < p; ++q) dup = dup || pair_of(q) == pair_of(p);
+    if(dup) continue;
+    h->grouped_policy = p;
     double ms = 0.0;
     st = time_ms(s, 3, [&] { return run_grouped(h, prec, problems, count, deps, ndeps, s); }, &ms);
-    if(st != MFX_SUCCESS) break;
-    if(best_ms < 0 || ms < best_ms) {
+    if(st == MFX_SUCCESS && (best < 0 || ms < best_ms)) {
+      best = p;
       best_ms = ms;
-      best = w;
     }
   }
+  h->grouped_policy = forced_policy;
   h->timing = timing;
-  h->grouped_waste = best;
-  if(st != MFX_SUCCESS) {
-    h->grouped_waste = saved;
-    return st;
-  }
-  // Record the winning policy's choice for every problem's shape bucket.
-  for(int i = 0; i < count; ++i)
-    record_tuned("g" + tune_key(prec, problems[i].m, problems[i].n, problems[i].k),
-        choose_grouped_config(h, prec, problems[i].m, problems[i].n, problems[i].k));
-  g_best_waste = best;
-  if(best_waste) *best_waste = best;
+  MFX_CHECK(st);
+  record_tuned(grouped_tune_key(prec, problems, count, deps, ndeps), best);
+  if(best_policy) *best_policy = best;
+  return MFX_SUCCESS;
+}
+
+int32_t mfx_grouped_policy_count(void) { return grouped_policy_count(); }
+
+mfx_status mfx_force_grouped_policy(mfx_handle handle, int32_t policy) {
+  if(!handle) return invalid("null handle");
+  if(policy < -1 || policy >= grouped_policy_count()) return invalid("grouped policy out of range");
+  handle->h.grouped_policy = policy;
+  return MFX_SUCCESS;
+}
+
+mfx_status mfx_grouped_policy_info(int32_t policy, int32_t* large_config, int32_t* small_config,
+    double* waste_bound) {
+  if(!large_config || !small_config || !waste_bound) return invalid("null output pointer");
+  int big, small;
+  double bound;
+  if(!grouped_policy_info(policy, &big, &small, &bound)) return invalid("grouped policy out of range");
+  *large_config = big;
+  *small_config = small;
+  *waste_bound = bound;
   return MFX_SUCCESS;
 }
 
@@ -184,7 +211,7 @@ mfx_status mfx_autotune_save(const char* path) {
     set_last_error(std::string("cannot write ") + path);
     return MFX_ERROR_IO;
   }
-  f << "{\n  \"format\": \"mfx-autotune-1\",\n  \"grouped_waste\": " << g_best_waste << ",\n  \"entries\": {";
+  f << "{\n  \"format\": \"mfx-autotune-2\",\n  \"entries\": {";
   bool first = true;
   for(const auto& kv : tuned_snapshot()) {
     f << (first ? "\n" : ",\n") << "    \"" << json_escape(kv.first) << "\": " << kv.second;
@@ -208,7 +235,7 @@ mfx_status mfx_autotune_load(const char* path) {
   std::stringstream ss;
   ss << f.rdbuf();
   const std::string text = ss.str();
-  if(text.find("\"mfx-autotune-1\"") == std::string::npos) {
+  if(text.find("\"mfx-autotune-2\"") == std::string::npos) {
     set_last_error(std::string(path) + " is not an mfx autotune file");
     return MFX_ERROR_IO;
   }
@@ -240,8 +267,6 @@ mfx_status mfx_autotune_load(const char* path) {
     pos = end - text.c_str();
   }
   replace_tuned(table);
-  const size_t gw = text.find("\"grouped_waste\"");
-  if(gw != std::string::npos) g_best_waste = std::strtod(text.c_str() + text.find(':', gw) + 1, nullptr);
   return MFX_SUCCESS;
 }
 

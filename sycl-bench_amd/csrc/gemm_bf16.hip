@@ -188,9 +188,10 @@ mfx_status launch_dense_bf16x(Handle* h, const DenseOp& op, bool split, int conf
 namespace {
 __global__ void split_elementwise_kernel(int64_t n, const float* x, uint16_t* hi, uint16_t* lo) {
   for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const uint16_t h = bf16_rne(x[i]);
-    hi[i] = h;
-    lo[i] = bf16_rne(x[i] - bf16_to_f32(h));
+    // the conversion the contraction kernels use (split_chunk)
+    const uint32_t h = cvt_pk_bf16(x[i], 0.0f) & 0xffffu;
+    hi[i] = (uint16_t)h;
+    lo[i] = (uint16_t)(cvt_pk_bf16(x[i] - bf16_to_f32((uint16_t)h), 0.0f) & 0xffffu);
   }
 }
 }  // namespace

@@ -17,7 +17,7 @@ struct F32Params {
   int vec_a, vec_b;  // 128-bit global loads allowed for A / B
 };
 
-template <int BM, int BN, int BK, int WM, int WN, bool AK, bool BKM>
+template <int BM, int BN, int BK, int WM, int WN, bool AK, bool BKM, bool VEC>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f32_kernel(const F32Params p) {
   __shared__ float smem[F32Cfg<BM, BN, BK, WM, WN>::SMEM_FLOATS];
   int tm, tn;
@@ -29,7 +29,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f32_kernel(co
     lower_tile(pid, ti, tj);
     if(p.op.tiles == TILES_LOWER) { tm = ti; tn = tj; } else { tm = tj; tn = ti; }
   }
-  f32_tile<BM, BN, BK, WM, WN, AK, BKM>(p.op, p.vec_a, p.vec_b, tm, tn, smem);
+  f32_tile<BM, BN, BK, WM, WN, AK, BKM, VEC>(p.op, tm, tn, smem);
 }
 
 struct F32Config {
@@ -44,15 +44,23 @@ constexpr F32Config kF32Configs[] = {
 };
 constexpr int kF32ConfigCount = sizeof(kF32Configs) / sizeof(kF32Configs[0]);
 
-template <int BM, int BN, int BK, int WM, int WN>
-mfx_status launch_cfg(Handle* h, const F32Params& p, int grid, hipStream_t s) {
+template <int BM, int BN, int BK, int WM, int WN, bool VEC>
+void launch_layout(const F32Params& p, int grid, hipStream_t s) {
   constexpr int NT = F32Cfg<BM, BN, BK, WM, WN>::NT;
   const bool ak = p.op.a_kmajor, bk = p.op.b_kmajor;
+  if(ak && !bk) gemm_f32_kernel<BM, BN, BK, WM, WN, true, false, VEC><<<grid, NT, 0, s>>>(p);
+  else if(ak && bk) gemm_f32_kernel<BM, BN, BK, WM, WN, true, true, VEC><<<grid, NT, 0, s>>>(p);
+  else if(!ak && !bk) gemm_f32_kernel<BM, BN, BK, WM, WN, false, false, VEC><<<grid, NT, 0, s>>>(p);
+  else gemm_f32_kernel<BM, BN, BK, WM, WN, false, true, VEC><<<grid, NT, 0, s>>>(p);
+}
+
+template <int BM, int BN, int BK, int WM, int WN>
+mfx_status launch_cfg(Handle* h, const F32Params& p, int grid, hipStream_t s) {
+  // 128-bit loads only when both operands allow them.
+  const bool vec = p.vec_a && p.vec_b;
   return timed_launch(h, "gemm_f32_mfma", s, [&] {
-    if(ak && !bk) gemm_f32_kernel<BM, BN, BK, WM, WN, true, false><<<grid, NT, 0, s>>>(p);
-    else if(ak && bk) gemm_f32_kernel<BM, BN, BK, WM, WN, true, true><<<grid, NT, 0, s>>>(p);
-    else if(!ak && !bk) gemm_f32_kernel<BM, BN, BK, WM, WN, false, false><<<grid, NT, 0, s>>>(p);
-    else gemm_f32_kernel<BM, BN, BK, WM, WN, false, true><<<grid, NT, 0, s>>>(p);
+    if(vec) launch_layout<BM, BN, BK, WM, WN, true>(p, grid, s);
+    else launch_layout<BM, BN, BK, WM, WN, false>(p, grid, s);
   });
 }
 

@@ -1,0 +1,440 @@
+// Block-scaled FP8 (OCP e4m3) contractions on v_mfma_scale_f32_32x32x64_f8f6f4.
+//
+// Quantisation (one workgroup per 128 x 128 block of an operand): the block's
+// largest magnitude fixes a power-of-two scale 2^e (the smallest e with
+// amax <= 448 * 2^e), every element becomes q = e4m3_rne(x * 2^-e).  Powers of
+// two make the scale an E8M0 byte that the MFMA applies itself, per operand,
+// so products need no rescaling pass.  Two-term mode quantises the residual
+// x - q1 * 2^e1 the same way (q2, e2) and contracts q1*q1 + q1*q2 + q2*q1.
+//
+// Packed layout (A role: rows = M; B role: rows = N), fragment order of the
+// 32x32x64 MFMA: P[blk32][kb64][half][lane][16 bytes], lane l holding row
+// blk32*32 + l%32 and k = kb64*64 + 32*(l/32) + 16*half + 0..15.  Rows and k
+// are zero-padded to the tile and to 128.  Exponents: E[row/128][k/128].
+//
+// Contraction tiles: 256x256 (waves of 128x128) or 128x128 (waves of 64x64);
+// each wave's rows and columns lie inside one 128-block, so its A and B
+// scales are one byte each per stage.  Stage = 128 k (single) or 64 k
+// (two-term), double-buffered in LDS, 16-byte chunks read conflict-free.
+#include "mfx_internal.h"
+
+namespace mfx {
+
+int fp8_config_count();
+bool fp8_config_info(int c, int* bm, int* bn, int* bk, int* waves);
+
+namespace {
+
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Quantisation
+// ---------------------------------------------------------------------------
+struct QuantParams {
+  const float* src;
+  int64_t ld;
+  int rows, K;
+  int kmajor;       // element (r, k) at src[r*ld + k], else src[k*ld + r]
+  int kb128s;       // padded K / 128
+  int two_term;
+  u32x4* q1;        // packed, fragment order
+  u32x4* q2;
+  int* e1;          // [rows_pad/128][kb128s]
+  int* e2;
+  // plain row-major output (mfx_quantize_fp8_block), used when q1 == nullptr
+  uint8_t* plain;
+  int64_t plain_ld;
+  int block;        // 128 for the contraction; 32/64/128 for the utility
+};
+
+__device__ __forceinline__ float block_max(float v, float* sh) {
+  for(int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  float m = sh[0];
+  for(int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmaxf(m, sh[i]);
+  return m;
+}
+
+// One workgroup per (row block, k block) of `block` x `block` elements;
+// thread t handles chunks of 16 consecutive k of one row.
+__global__ __launch_bounds__(256) void quantize_fp8_kernel(const QuantParams p) {
+  __shared__ float sh[4];
+  const int rb = blockIdx.y, kb = blockIdx.x;
+  const int B = p.block;
+  const int per_row = B / 16;
+  const int chunks = B * per_row;  // <= 1024
+  float v[4][16];
+  float amax = 0.0f;
+#pragma unroll
+  for(int c = 0; c < 4; ++c) {
+    const int id = threadIdx.x + c * 256;
+    const int r = rb * B + id / per_row, k0 = kb * B + (id % per_row) * 16;
+#pragma unroll
+    for(int e = 0; e < 16; ++e) {
+      float x = 0.0f;
+      if(id < chunks && r < p.rows && k0 + e < p.K)
+        x = p.kmajor ? p.src[(int64_t)r * p.ld + k0 + e] : p.src[(int64_t)(k0 + e) * p.ld + r];
+      v[c][e] = x;
+      amax = fmaxf(amax, fabsf(x));
+    }
+  }
+  const int ex1 = fp8_block_exponent(block_max(amax, sh));
+  const float inv1 = ldexpf(1.0f, -ex1);
+  uint8_t q[4][16];
+  float amax2 = 0.0f;
+#pragma unroll
+  for(int c = 0; c < 4; ++c)
+#pragma unroll
+    for(int e = 0; e < 16; ++e) {
+      q[c][e] = e4m3_rne_sat(v[c][e] * inv1);
+      if(p.two_term) {
+        v[c][e] -= ldexpf(e4m3_to_f32(q[c][e]), ex1);
+        amax2 = fmaxf(amax2, fabsf(v[c][e]));
+      }
+    }
+  int ex2 = 0;
+  float inv2 = 1.0f;
+  if(p.two_term) {
+    ex2 = fp8_block_exponent(block_max(amax2, sh));
+    inv2 = ldexpf(1.0f, -ex2);
+  }
+  if(threadIdx.x == 0) {
+    p.e1[rb * p.kb128s + kb] = ex1;
+    if(p.two_term) p.e2[rb * p.kb128s + kb] = ex2;
+  }
+#pragma unroll
+  for(int c = 0; c < 4; ++c) {
+    const int id = threadIdx.x + c * 256;
+    if(id >= chunks) continue;
+    const int r = rb * B + id / per_row, k0 = kb * B + (id % per_row) * 16;
+    uint32_t w1[4], w2[4];
+#pragma unroll
+    for(int j = 0; j < 4; ++j) {
+      w1[j] = (uint32_t)q[c][4 * j] | ((uint32_t)q[c][4 * j + 1] << 8) | ((uint32_t)q[c][4 * j + 2] << 16) |
+              ((uint32_t)q[c][4 * j + 3] << 24);
+      if(p.two_term) {
+        uint8_t b[4];
+        for(int t = 0; t < 4; ++t) b[t] = e4m3_rne_sat(v[c][4 * j + t] * inv2);
+        w2[j] = (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24);
+      }
+    }
+    if(p.q1) {
+      // fragment-order position of (r, k0..k0+15)
+      const int kb64s = p.kb128s * 2;
+      const int blk = r / 32, kb64 = k0 / 64, w64 = k0 % 64;
+      const int lane = (r % 32) + 32 * (w64 / 32), half = (w64 % 32) / 16;
+      const int64_t at = (((int64_t)blk * kb64s + kb64) * 2 + half) * 64 + lane;
+      u32x4 c1;
+      c1.x = w1[0];
+      c1.y = w1[1];
+      c1.z = w1[2];
+      c1.w = w1[3];
+      p.q1[at] = c1;
+      if(p.two_term) {
+        u32x4 c2;
+        c2.x = w2[0];
+        c2.y = w2[1];
+        c2.z = w2[2];
+        c2.w = w2[3];
+        p.q2[at] = c2;
+      }
+    } else if(r < p.rows) {
+      for(int e = 0; e < 16 && k0 + e < p.K; ++e) p.plain[(int64_t)r * p.plain_ld + k0 + e] = q[c][e];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Contraction
+// ---------------------------------------------------------------------------
+struct Fp8TileArgs {
+  const u32x4* A1;
+  const u32x4* A2;
+  const u32x4* B1;
+  const u32x4* B2;
+  const int* eA1;
+  const int* eA2;
+  const int* eB1;
+  const int* eB2;
+  int kb128s;  // padded K / 128
+  int tiles_m, tiles_n, group;
+  Epilogue ep;
+};
+
+template <int BM, int BN, int WM, int WN, int T>
+struct Fp8Cfg {
+  static constexpr int NWN = BN / WN, NT = (BM / WM) * (BN / WN) * 64;
+  static constexpr int FM = WM / 32, FN = WN / 32;
+  static constexpr int KS = T == 1 ? 2 : 1;                   // 64-wide k steps per stage
+  static constexpr int RUN = KS * 2 * 64;                     // chunks of one 32-row block per stage and term
+  static constexpr int CH_A = BM / 32 * RUN * T, CH_B = BN / 32 * RUN * T;
+  static constexpr int PER_A = CH_A / NT, PER_B = CH_B / NT;
+  static_assert(CH_A % NT == 0 && CH_B % NT == 0, "stage must split evenly");
+  static_assert(WM <= 128 && 128 % WM == 0 && WN <= 128 && 128 % WN == 0, "a wave must sit in one scale block");
+};
+
+__device__ __forceinline__ i32x8 join(u32x4 lo, u32x4 hi) {
+  i32x8 r;
+  r[0] = lo.x; r[1] = lo.y; r[2] = lo.z; r[3] = lo.w;
+  r[4] = hi.x; r[5] = hi.y; r[6] = hi.z; r[7] = hi.w;
+  return r;
+}
+
+template <int BM, int BN, int WM, int WN, int T>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_fp8_kernel(const Fp8TileArgs t) {
+  using Cfg = Fp8Cfg<BM, BN, WM, WN, T>;
+  constexpr int NT = Cfg::NT, FM = Cfg::FM, FN = Cfg::FN, KS = Cfg::KS, RUN = Cfg::RUN;
+  constexpr int SA = Cfg::CH_A, SB = Cfg::CH_B;
+  __shared__ u32x4 lds[2 * (SA + SB)];
+
+  int tm, tn;
+  grouped_tile(xcd_remap(blockIdx.x, gridDim.x), t.tiles_m, t.tiles_n, t.group, tm, tn);
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm = wid / Cfg::NWN, wn = wid % Cfg::NWN;
+  const int arow128 = (tm * BM + wm * WM) / 128, bcol128 = (tn * BN + wn * WN) / 128;
+
+  floatx16 acc[FM][FN];
+#pragma unroll
+  for(int i = 0; i < FM; ++i)
+#pragma unroll
+    for(int j = 0; j < FN; ++j)
+#pragma unroll
+      for(int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
+
+  const int kb64s = t.kb128s * 2;
+  const int steps = T == 1 ? t.kb128s : kb64s;
+  const int64_t blk_stride = (int64_t)kb64s * 2 * 64;
+  u32x4 ra[Cfg::PER_A], rb[Cfg::PER_B];
+
+  auto issue = [&](int s) __attribute__((always_inline)) {
+    const int64_t koff = (int64_t)s * RUN;
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_A; ++i) {
+      const int c = tid + i * NT;
+      const int term = c / (SA / T), cc = c % (SA / T);
+      const int blk = cc / RUN, run = cc - blk * RUN;
+      const u32x4* src = term ? t.A2 : t.A1;
+      ra[i] = src[(int64_t)(tm * (BM / 32) + blk) * blk_stride + koff + run];
+    }
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_B; ++i) {
+      const int c = tid + i * NT;
+      const int term = c / (SB / T), cc = c % (SB / T);
+      const int blk = cc / RUN, run = cc - blk * RUN;
+      const u32x4* src = term ? t.B2 : t.B1;
+      rb[i] = src[(int64_t)(tn * (BN / 32) + blk) * blk_stride + koff + run];
+    }
+  };
+  auto commit = [&](int buf) __attribute__((always_inline)) {
+    u32x4* sa = lds + buf * (SA + SB);
+    u32x4* sb = sa + SA;
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_A; ++i) sa[tid + i * NT] = ra[i];
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_B; ++i) sb[tid + i * NT] = rb[i];
+  };
+  auto compute = [&](int s, int cur) __attribute__((always_inline)) {
+    const u32x4* sa = lds + cur * (SA + SB);
+    const u32x4* sb = sa + SA;
+    const int kb128 = T == 1 ? s : s / 2;
+    // E8M0 scale bytes (biased exponents) of this wave's A rows and B columns
+    const int sA1 = t.eA1[arow128 * t.kb128s + kb128] + 127, sB1 = t.eB1[bcol128 * t.kb128s + kb128] + 127;
+    int sA2 = 0, sB2 = 0;
+    if(T == 2) {
+      sA2 = t.eA2[arow128 * t.kb128s + kb128] + 127;
+      sB2 = t.eB2[bcol128 * t.kb128s + kb128] + 127;
+    }
+#pragma unroll
+    for(int ks = 0; ks < KS; ++ks) {
+      i32x8 a1[FM], a2[FM], b1[FN], b2[FN];
+#pragma unroll
+      for(int i = 0; i < FM; ++i) {
+        const int base = (wm * FM + i) * RUN + ks * 128 + lane;
+        a1[i] = join(sa[base], sa[base + 64]);
+        if(T == 2) a2[i] = join(sa[SA / 2 + base], sa[SA / 2 + base + 64]);
+      }
+#pragma unroll
+      for(int j = 0; j < FN; ++j) {
+        const int base = (wn * FN + j) * RUN + ks * 128 + lane;
+        b1[j] = join(sb[base], sb[base + 64]);
+        if(T == 2) b2[j] = join(sb[SB / 2 + base], sb[SB / 2 + base + 64]);
+      }
+      if(T == 2) {
+#pragma unroll
+        for(int i = 0; i < FM; ++i)
+#pragma unroll
+          for(int j = 0; j < FN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a2[i], b1[j], acc[i][j], 0, 0, 0, sA2, 0, sB1);
+#pragma unroll
+        for(int i = 0; i < FM; ++i)
+#pragma unroll
+          for(int j = 0; j < FN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a1[i], b2[j], acc[i][j], 0, 0, 0, sA1, 0, sB2);
+      }
+#pragma unroll
+      for(int i = 0; i < FM; ++i)
+#pragma unroll
+        for(int j = 0; j < FN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a1[i], b1[j], acc[i][j], 0, 0, 0, sA1, 0, sB1);
+    }
+  };
+
+  if(steps > 0) {
+    issue(0);
+    commit(0);
+  }
+  __syncthreads();
+  for(int s = 0; s + 1 < steps; ++s) {
+    issue(s + 1);
+    compute(s, s & 1);
+    commit((s + 1) & 1);
+    __syncthreads();
+  }
+  if(steps > 0) compute(steps - 1, (steps - 1) & 1);
+
+  static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
+    constexpr int i = decltype(ij)::value / FN, j = decltype(ij)::value % FN;
+    store_block32(t.ep, acc[i][j], tm * BM + wm * WM + i * 32, tn * BN + wn * WN + j * 32, lane);
+  });
+}
+
+struct Fp8Config {
+  int bm, bn;
+};
+constexpr Fp8Config kFp8Configs[] = {{256, 256}, {128, 128}};
+constexpr int kFp8ConfigCount = 2;
+
+struct Fp8Plan {
+  int64_t rows_pad_a, rows_pad_b, kb128s;
+  size_t q_a, q_b, e_a, e_b;  // bytes of one term
+};
+
+Fp8Plan fp8_plan(const DenseOp& op, int config) {
+  const Fp8Config& c = kFp8Configs[config];
+  Fp8Plan pl;
+  pl.rows_pad_a = round_up(op.M, std::max(c.bm, 128));
+  pl.rows_pad_b = round_up(op.N, std::max(c.bn, 128));
+  pl.kb128s = round_up(std::max(op.K, 1), 128) / 128;
+  pl.q_a = (size_t)pl.rows_pad_a * pl.kb128s * 128;
+  pl.q_b = (size_t)pl.rows_pad_b * pl.kb128s * 128;
+  pl.e_a = (size_t)(pl.rows_pad_a / 128) * pl.kb128s * 4;
+  pl.e_b = (size_t)(pl.rows_pad_b / 128) * pl.kb128s * 4;
+  return pl;
+}
+
+mfx_status quantize(Handle* h, const float* src, int64_t ld, int rows, int K, bool kmajor, int64_t rows_pad,
+    int64_t kb128s, bool two, u32x4* q1, u32x4* q2, int* e1, int* e2, hipStream_t s) {
+  QuantParams p{};
+  p.src = src;
+  p.ld = ld;
+  p.rows = rows;
+  p.K = K;
+  p.kmajor = kmajor;
+  p.kb128s = (int)kb128s;
+  p.two_term = two;
+  p.q1 = q1;
+  p.q2 = q2;
+  p.e1 = e1;
+  p.e2 = e2;
+  p.block = 128;
+  dim3 grid((unsigned)kb128s, (unsigned)(rows_pad / 128));
+  return timed_launch(h, two ? "quantize_fp8_block2" : "quantize_fp8_block", s,
+      [&] { quantize_fp8_kernel<<<grid, 256, 0, s>>>(p); });
+}
+
+}  // namespace
+
+int fp8_config_count() { return kFp8ConfigCount; }
+bool fp8_config_info(int c, int* bm, int* bn, int* bk, int* waves) {
+  if(c < 0 || c >= kFp8ConfigCount) return false;
+  *bm = kFp8Configs[c].bm;
+  *bn = kFp8Configs[c].bn;
+  *bk = 128;
+  *waves = 4;
+  return true;
+}
+
+size_t dense_fp8_workspace(const DenseOp& op, bool two_term, int config) {
+  if(config < 0 || config >= kFp8ConfigCount) return 0;
+  const Fp8Plan pl = fp8_plan(op, config);
+  const int terms = two_term ? 2 : 1;
+  return terms * (round_up(pl.q_a, 256) + round_up(pl.q_b, 256) + round_up(pl.e_a, 256) + round_up(pl.e_b, 256));
+}
+
+mfx_status launch_dense_fp8(Handle* h, const DenseOp& op, bool two_term, int config, hipStream_t s) {
+  if(config < 0 || config >= kFp8ConfigCount) return invalid("FP8 tile configuration out of range");
+  if(op.nprod != 1 || op.tiles != TILES_FULL) return unsupported("FP8 is built for plain gemm (not syrk / syr2k)");
+  if(op.M == 0 || op.N == 0) return MFX_SUCCESS;
+  const Fp8Plan pl = fp8_plan(op, config);
+  void* ws = nullptr;
+  MFX_CHECK(workspace(h, dense_fp8_workspace(op, two_term, config), &ws));
+  char* w = static_cast<char*>(ws);
+  auto take = [&](size_t bytes) {
+    char* p = w;
+    w += round_up(bytes, 256);
+    return p;
+  };
+  u32x4* qa1 = reinterpret_cast<u32x4*>(take(pl.q_a));
+  u32x4* qb1 = reinterpret_cast<u32x4*>(take(pl.q_b));
+  int* ea1 = reinterpret_cast<int*>(take(pl.e_a));
+  int* eb1 = reinterpret_cast<int*>(take(pl.e_b));
+  u32x4 *qa2 = nullptr, *qb2 = nullptr;
+  int *ea2 = nullptr, *eb2 = nullptr;
+  if(two_term) {
+    qa2 = reinterpret_cast<u32x4*>(take(pl.q_a));
+    qb2 = reinterpret_cast<u32x4*>(take(pl.q_b));
+    ea2 = reinterpret_cast<int*>(take(pl.e_a));
+    eb2 = reinterpret_cast<int*>(take(pl.e_b));
+  }
+  MFX_CHECK(quantize(h, op.A, op.lda, op.M, op.K, op.a_kmajor, pl.rows_pad_a, pl.kb128s, two_term, qa1, qa2, ea1, ea2, s));
+  MFX_CHECK(quantize(h, op.B, op.ldb, op.N, op.K, op.b_kmajor, pl.rows_pad_b, pl.kb128s, two_term, qb1, qb2, eb1, eb2, s));
+  Fp8TileArgs t;
+  t.A1 = qa1;
+  t.A2 = qa2 ? qa2 : qa1;
+  t.B1 = qb1;
+  t.B2 = qb2 ? qb2 : qb1;
+  t.eA1 = ea1;
+  t.eA2 = ea2 ? ea2 : ea1;
+  t.eB1 = eb1;
+  t.eB2 = eb2 ? eb2 : eb1;
+  t.kb128s = (int)pl.kb128s;
+  t.ep = op.ep;
+  t.group = 8;
+  const Fp8Config& c = kFp8Configs[config];
+  t.tiles_m = (int)ceil_div(op.M, c.bm);
+  t.tiles_n = (int)ceil_div(op.N, c.bn);
+  const int grid = t.tiles_m * t.tiles_n;
+  const char* name = two_term ? "gemm_fp8_block2_mfma" : "gemm_fp8_block_mfma";
+  if(config == 0) {
+    if(two_term)
+      return timed_launch(h, name, s, [&] { gemm_fp8_kernel<256, 256, 128, 128, 2><<<grid, 256, 0, s>>>(t); });
+    return timed_launch(h, name, s, [&] { gemm_fp8_kernel<256, 256, 128, 128, 1><<<grid, 256, 0, s>>>(t); });
+  }
+  if(two_term) return timed_launch(h, name, s, [&] { gemm_fp8_kernel<128, 128, 64, 64, 2><<<grid, 256, 0, s>>>(t); });
+  return timed_launch(h, name, s, [&] { gemm_fp8_kernel<128, 128, 64, 64, 1><<<grid, 256, 0, s>>>(t); });
+}
+
+mfx_status launch_quantize_fp8(Handle* h, int64_t rows, int64_t cols, const float* x, int64_t ldx, int block,
+    uint8_t* q, int32_t* exps, hipStream_t s) {
+  if(rows == 0 || cols == 0) return MFX_SUCCESS;
+  QuantParams p{};
+  p.src = x;
+  p.ld = ldx;
+  p.rows = (int)rows;
+  p.K = (int)cols;
+  p.kmajor = 1;
+  p.kb128s = (int)ceil_div(cols, block);
+  p.two_term = 0;
+  p.e1 = exps;
+  p.plain = q;
+  p.plain_ld = cols;
+  p.block = block;
+  dim3 grid((unsigned)ceil_div(cols, block), (unsigned)ceil_div(rows, block));
+  return timed_launch(h, "quantize_fp8_block_plain", s, [&] { quantize_fp8_kernel<<<grid, 256, 0, s>>>(p); });
+}
+
+}  // namespace mfx

@@ -1,226 +1,142 @@
-// Persistent grouped launch: many uneven contractions, and chains of them
-// (2mm, 3mm, matmulchain), in one kernel launch.
-//
-// The host turns the problem list into a work queue of items in topological
-// order:
-//   PACK_A / PACK_B (split / single BF16 only): convert one tile-row panel of
-//       an operand into fragment order (tile_bf16.h) in the workspace;
-//   TILE: one output tile of one problem.
-// Level by level (a problem's level is 1 + the highest level of its
-// producers): the level's pack items, then its tile items, largest K first.
-// A fixed grid of one workgroup per CU pulls items from an atomic head
-// counter.  Before working on an item it waits, with a bounded spin, for
-//   - PACK: every producer of the problem to have finished all its tiles;
-//   - TILE: its A and B panels to be packed (BF16 modes), or its producers to
-//     be finished (FP32, which reads the operands directly).
-// Completion is published by fence + barrier + atomic increment; waits are
-// acquire loads at agent scope, so producer results written on another XCD
-// are visible.  Deadlock freedom: a workgroup only waits for items that
-// precede its own in the queue, and each of those was taken by a workgroup
-// that is running and waits only for items before it.  A wait that exceeds
-// its spin limit sets an error flag and skips the item; every workgroup still
-// drains the queue and exits.
+// Host side of the persistent grouped launch (queue layout and protocol in
+// grouped.h): planning, workspace binding, the tile policy, chains, the
+// hipGraph form and the C entry points.
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <numeric>
 
-#include "tile_bf16.h"
-#include "tile_f32.h"
+#include "grouped.h"
 
 namespace mfx {
 
-int choose_grouped_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k);
+std::string tune_key(mfx_precision p, int64_t m, int64_t n, int64_t k);
+int tuned_lookup(const std::string& key);
 
 namespace {
 
-constexpr int MAX_DEPS = 4;
-enum ItemType : int { ITEM_PACK_A = 0, ITEM_PACK_B = 1, ITEM_TILE = 2 };
-
-struct GProblem {
-  int M, N, K;
-  float alpha, beta;
-  const float* A;
-  int64_t lda;
-  const float* B;
-  int64_t ldb;
-  float* C;
-  int64_t ldc;
-  int a_kmajor, b_kmajor;
-  int vec_a, vec_b;
-  int config;
-  int tiles_m, tiles_n;
-  int kbs;                 // BF16: padded K / 16
-  u32x4* packed_a;         // BF16: fragment-order operands
-  u32x4* packed_b;
-  int pack_a_flags;        // offset of this problem's A-panel flags in the flag array
-  int pack_b_flags;
-  int ndeps;
-  int deps[MAX_DEPS];      // producer problem indices
+// Tile policies: (large, small, bound) -- a problem takes the large tile when
+// that pads its output area by at most `bound` (0: always), else the small
+// one.  The grouped kernels do not compile the 256x256 configuration (0).
+struct GroupedPolicy {
+  int big, small;
+  double bound;
 };
-
-struct GItem {
-  int type, problem, x, y;  // PACK: x = panel; TILE: (x, y) = (tm, tn)
+constexpr GroupedPolicy kPolicies[] = {
+    {3, 3, 0.0},   // 0
+    {3, 1, 1.34},  // 1
+    {3, 1, 2.0},   // 2
+    {3, 2, 1.34},  // 3
+    {1, 1, 0.0},   // 4
+    {1, 2, 1.34},  // 5
+    {2, 2, 0.0},   // 6
 };
+constexpr int kPolicyCount = sizeof(kPolicies) / sizeof(kPolicies[0]);
 
-struct GroupedArgs {
-  const GProblem* problems;
-  const GItem* items;
-  int nitems;
-  int* head;        // next item to take
-  int* error;       // set to 1 when a wait times out
-  int* done;        // per problem: finished tiles
-  int* pack_flags;  // per packed panel: 1 when packed
-  int spin_limit;
-};
+// Relative per-CU contraction rate of each tile configuration on large
+// problems (larger tiles reuse each loaded panel more), used by the
+// untuned policy estimate.
+constexpr double kTileRate[4] = {1.0, 0.6, 0.2, 0.85};
 
-// Bounded wait for *ctr >= target (all threads of the workgroup).
-__device__ __forceinline__ bool wait_at_least(int* ctr, int target, const GroupedArgs& g) {
-  int spins = 0;
-  while(__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-    if(++spins > g.spin_limit) {
-      __hip_atomic_store(g.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return false;
-    }
-    __builtin_amdgcn_s_sleep(8);
+int problem_config(const GroupedPolicy& pol, int64_t m, int64_t n) {
+  const int big = pol.big, small = pol.small;
+  if(big == small || pol.bound <= 0.0) return big;
+  const double padded = (double)round_up(m, host_bm(big)) * (double)round_up(n, host_bn(big));
+  return padded <= pol.bound * (double)m * (double)n ? big : small;
+}
+
+// Topological level of every problem (producers come first, so one pass).
+std::vector<int> problem_levels(int count, const mfx_dep* deps, int ndeps) {
+  std::vector<int> level(count, 0);
+  std::vector<std::vector<int>> prod(count);
+  for(int d = 0; d < ndeps; ++d) prod[deps[d].consumer].push_back(deps[d].producer);
+  for(int i = 0; i < count; ++i)
+    for(int p : prod[i]) level[i] = std::max(level[i], level[p] + 1);
+  return level;
+}
+
+// Estimated duration (arbitrary units) of a launch under a policy: per level,
+// the padded tile work spread over the CUs plus half a tile of tail.
+double policy_estimate(const GroupedPolicy& pol, const mfx_gemm_problem* problems, int count,
+    const std::vector<int>& level, int cus) {
+  const int levels = count ? *std::max_element(level.begin(), level.end()) + 1 : 0;
+  std::vector<double> work(levels, 0.0), tmax(levels, 0.0);
+  std::vector<int64_t> ntiles(levels, 0);
+  for(int i = 0; i < count; ++i) {
+    const auto& q = problems[i];
+    if(q.m <= 0 || q.n <= 0) continue;
+    const int c = problem_config(pol, q.m, q.n);
+    const int64_t tiles = ceil_div(q.m, host_bm(c)) * ceil_div(q.n, host_bn(c));
+    const double t = (double)host_bm(c) * host_bn(c) * (double)round_up(std::max<int64_t>(q.k, 1), 32) / kTileRate[c];
+    work[level[i]] += t * tiles;
+    tmax[level[i]] = std::max(tmax[level[i]], t);
+    ntiles[level[i]] += tiles;
   }
+  double est = 0.0;
+  for(int l = 0; l < levels; ++l) {
+    if(ntiles[l] == 0) continue;
+    est += std::max(work[l] / cus + 0.5 * work[l] / ntiles[l], tmax[l]);
+  }
+  return est;
+}
+
+}  // namespace
+
+int grouped_policy_count() { return kPolicyCount; }
+
+bool grouped_policy_info(int id, int* big, int* small, double* bound) {
+  if(id < 0 || id >= kPolicyCount) return false;
+  *big = kPolicies[id].big;
+  *small = kPolicies[id].small;
+  *bound = kPolicies[id].bound;
   return true;
 }
 
-__device__ __forceinline__ bool wait_producers(const GProblem& P, const GroupedArgs& g) {
-  bool ok = true;
-  for(int d = 0; d < P.ndeps; ++d) {
-    const GProblem& Q = g.problems[P.deps[d]];
-    ok = wait_at_least(&g.done[P.deps[d]], Q.tiles_m * Q.tiles_n, g) && ok;
+// Tuning-table key of a grouped launch: precision, problem count, total
+// contraction volume and dependency depth, each bucketed by powers of two.
+std::string grouped_tune_key(mfx_precision p, const mfx_gemm_problem* problems, int count, const mfx_dep* deps,
+    int ndeps) {
+  double vol = 0.0;
+  for(int i = 0; i < count; ++i) vol += (double)problems[i].m * problems[i].n * problems[i].k;
+  const auto level = problem_levels(count, deps, ndeps);
+  const int levels = count ? *std::max_element(level.begin(), level.end()) + 1 : 0;
+  return "g" + tune_key(p, count, (int64_t)std::min(std::sqrt(vol), 1e18), levels);
+}
+
+// Policy of a grouped launch: the tuner's override, a forced configuration,
+// a tuned entry, or the estimate.
+int choose_grouped_policy(Handle* h, mfx_precision prec, const mfx_gemm_problem* problems, int count,
+    const mfx_dep* deps, int ndeps) {
+  if(h->grouped_policy >= 0) return h->grouped_policy;
+  int forced = h->forced_config[prec];
+  if(forced >= 0) {
+    if(forced == 0) forced = 3;  // no grouped 256x256 kernel
+    for(int i = 0; i < kPolicyCount; ++i)
+      if(kPolicies[i].big == forced && kPolicies[i].small == forced) return i;
   }
-  return ok;
-}
-
-// Publishes the workgroup's global writes, then bumps *ctr once.
-__device__ __forceinline__ void signal(int* ctr) {
-  __threadfence();
-  __syncthreads();
-  if(threadIdx.x == 0) __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ int take_item(int* head, int* slot) {
-  __syncthreads();  // everyone is done with the previous item (and its LDS)
-  if(threadIdx.x == 0) *slot = atomicAdd(head, 1);
-  __syncthreads();
-  return *slot;
-}
-
-__device__ __forceinline__ int tile_bm(int config) { return config == 1 ? 128 : config == 2 ? 64 : 256; }
-__device__ __forceinline__ int tile_bn(int config) { return config == 0 ? 256 : config == 2 ? 64 : 128; }
-
-template <int NHL>
-__device__ void pack_panel(const GProblem& P, bool is_a, int panel) {
-  const int rows_per_panel = is_a ? tile_bm(P.config) : tile_bn(P.config);
-  PackSrc s;
-  s.src = is_a ? P.A : P.B;
-  s.ld = is_a ? P.lda : P.ldb;
-  s.rows = is_a ? P.M : P.N;
-  s.K = P.K;
-  s.kmajor = is_a ? P.a_kmajor : P.b_kmajor;
-  s.vec = is_a ? P.vec_a : P.vec_b;
-  u32x4* dst = is_a ? P.packed_a : P.packed_b;
-  const int blk0 = panel * rows_per_panel / 32;
-  const int chunks = rows_per_panel / 32 * P.kbs * 64;
-  for(int c = threadIdx.x; c < chunks; c += 256) {
-    const int lane = c & 63, t = c >> 6;
-    pack_chunk<NHL>(s, blk0 + t / P.kbs, t % P.kbs, lane, P.kbs, dst);
-  }
-}
-
-template <int NHL>
-__global__ __launch_bounds__(256) void grouped_bf16x_kernel(const GroupedArgs g) {
-  __shared__ u32x4 lds[BfCfg<256, 256, 128, 128, NHL>::SMEM_CHUNKS];
-  __shared__ int slot;
-  for(;;) {
-    const int it = take_item(g.head, &slot);
-    if(it >= g.nitems) break;
-    const GItem item = g.items[it];
-    const GProblem& P = g.problems[item.problem];
-    if(item.type != ITEM_TILE) {
-      const bool is_a = item.type == ITEM_PACK_A;
-      if(!wait_producers(P, g)) continue;
-      pack_panel<NHL>(P, is_a, item.x);
-      signal(&g.pack_flags[(is_a ? P.pack_a_flags : P.pack_b_flags) + item.x]);
-      continue;
+  const int tuned = tuned_lookup(grouped_tune_key(prec, problems, count, deps, ndeps));
+  if(tuned >= 0 && tuned < kPolicyCount) return tuned;
+  const auto level = problem_levels(count, deps, ndeps);
+  const int cus = h->cu_count > 0 ? h->cu_count : 256;
+  int best = 0;
+  double best_est = -1.0;
+  for(int i = 0; i < kPolicyCount; ++i) {
+    const double e = policy_estimate(kPolicies[i], problems, count, level, cus);
+    if(best_est < 0 || e < best_est) {
+      best_est = e;
+      best = i;
     }
-    const bool ready = wait_at_least(&g.pack_flags[P.pack_a_flags + item.x], 1, g) &&
-                       wait_at_least(&g.pack_flags[P.pack_b_flags + item.y], 1, g);
-    if(!ready) continue;
-    BfTileArgs t;
-    t.PA = P.packed_a;
-    t.PB = P.packed_b;
-    t.kbs = P.kbs;
-    t.nprod = 1;
-    t.ep = Epilogue{P.C, P.ldc, P.alpha, P.beta, P.M, P.N, TILES_FULL, 0};
-    switch(P.config) {
-      case 0: bf16_tile<256, 256, 128, 128, NHL>(t, item.x, item.y, lds); break;
-      case 1: bf16_tile<128, 128, 64, 64, NHL>(t, item.x, item.y, lds); break;
-      case 2: bf16_tile<64, 64, 32, 32, NHL>(t, item.x, item.y, lds); break;
-      default: bf16_tile<256, 128, 128, 64, NHL>(t, item.x, item.y, lds); break;
-    }
-    signal(&g.done[item.problem]);
   }
+  return best;
 }
 
-template <int BM, int BN, int BK, int WM, int WN>
-__device__ __forceinline__ void f32_tile_any(const DenseOp& op, const GProblem& P, int tm, int tn, float* smem) {
-  if(P.a_kmajor && !P.b_kmajor) f32_tile<BM, BN, BK, WM, WN, true, false>(op, P.vec_a, P.vec_b, tm, tn, smem);
-  else if(P.a_kmajor) f32_tile<BM, BN, BK, WM, WN, true, true>(op, P.vec_a, P.vec_b, tm, tn, smem);
-  else if(!P.b_kmajor) f32_tile<BM, BN, BK, WM, WN, false, false>(op, P.vec_a, P.vec_b, tm, tn, smem);
-  else f32_tile<BM, BN, BK, WM, WN, false, true>(op, P.vec_a, P.vec_b, tm, tn, smem);
-}
-
-__global__ __launch_bounds__(256) void grouped_f32_kernel(const GroupedArgs g) {
-  __shared__ float smem[F32Cfg<256, 256, 16, 128, 128>::SMEM_FLOATS];
-  __shared__ int slot;
-  for(;;) {
-    const int it = take_item(g.head, &slot);
-    if(it >= g.nitems) break;
-    const GItem item = g.items[it];
-    const GProblem& P = g.problems[item.problem];
-    if(!wait_producers(P, g)) continue;
-    DenseOp op{};
-    op.M = P.M;
-    op.N = P.N;
-    op.K = P.K;
-    op.A = P.A;
-    op.lda = P.lda;
-    op.a_kmajor = P.a_kmajor;
-    op.B = P.B;
-    op.ldb = P.ldb;
-    op.b_kmajor = P.b_kmajor;
-    op.nprod = 1;
-    op.tiles = TILES_FULL;
-    op.ep = Epilogue{P.C, P.ldc, P.alpha, P.beta, P.M, P.N, TILES_FULL, 0};
-    switch(P.config) {
-      case 0: f32_tile_any<256, 256, 16, 128, 128>(op, P, item.x, item.y, smem); break;
-      case 1: f32_tile_any<128, 128, 16, 64, 64>(op, P, item.x, item.y, smem); break;
-      case 2: f32_tile_any<64, 64, 32, 32, 32>(op, P, item.x, item.y, smem); break;
-      default: f32_tile_any<256, 128, 16, 128, 64>(op, P, item.x, item.y, smem); break;
-    }
-    signal(&g.done[item.problem]);
-  }
-}
+namespace {
 
 // Clears the queue counters and the error flag before each launch (a kernel,
 // so the reset can be part of a captured graph).
 __global__ void zero_ints_kernel(int* p, int n, int* err) {
   for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
   if(blockIdx.x == 0 && threadIdx.x == 0) *err = 0;
-}
-
-int host_bm(int config) { return config == 1 ? 128 : config == 2 ? 64 : 256; }
-int host_bn(int config) { return config == 0 ? 256 : config == 2 ? 64 : 128; }
-
-// Bytes of the packed A and B copies of one problem (each rounded to 256).
-std::pair<size_t, size_t> packed_sizes(const GProblem& P, int nhl) {
-  const size_t ba = (size_t)P.tiles_m * host_bm(P.config) * P.kbs * 16 * nhl * 2;
-  const size_t bb = (size_t)P.tiles_n * host_bn(P.config) * P.kbs * 16 * nhl * 2;
-  return {(size_t)round_up(ba, 256), (size_t)round_up(bb, 256)};
 }
 
 }  // namespace
@@ -231,10 +147,11 @@ struct GroupedPlan {
   mfx_precision prec;
   std::vector<GProblem> problems;
   std::vector<GItem> items;
-  size_t table_bytes = 0, counter_ints = 0, packed_bytes = 0;
-  size_t off_problems = 0, off_items = 0, off_counters = 0, off_packed = 0;
+  size_t counter_ints = 0;
+  size_t off_problems = 0, off_items = 0, off_counters = 0;
   size_t total_bytes = 0;
-  int n_flags = 0;
+  int policy = 0, big = 0, small = 0;  // tile policy and its configurations
+  bool all_vec = true;                 // every problem allows 128-bit operand loads
   double flops = 0.0;
 };
 
@@ -274,8 +191,6 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
     return unsupported("grouped launch supports FP32, SPLIT_BF16 and BF16");
   MFX_CHECK(validate(problems, count, deps, ndeps));
   pl.prec = prec;
-  const bool packed = prec != MFX_PREC_FP32;
-  const int nhl = prec == MFX_PREC_SPLIT_BF16 ? 2 : 1;
   pl.problems.resize(count);
   std::vector<int> level(count, 0);
   for(int i = 0; i < count; ++i) pl.problems[i].ndeps = 0;
@@ -284,14 +199,16 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
     bool dup = false;
     for(int e = 0; e < c.ndeps; ++e) dup |= c.deps[e] == deps[d].producer;
     if(dup) continue;
-    if(c.ndeps == MAX_DEPS) return unsupported("a problem may depend on at most 4 producers");
+    if(c.ndeps == GROUPED_MAX_DEPS) return unsupported("a problem may depend on at most 4 producers");
     c.deps[c.ndeps++] = deps[d].producer;
   }
   for(int i = 0; i < count; ++i)
     for(int e = 0; e < pl.problems[i].ndeps; ++e) level[i] = std::max(level[i], level[pl.problems[i].deps[e]] + 1);
+  pl.policy = choose_grouped_policy(h, prec, problems, count, deps, ndeps);
+  const GroupedPolicy& pol = kPolicies[pl.policy];
+  pl.big = pol.big;
+  pl.small = pol.small;
 
-  size_t packed_bytes = 0;
-  int flags = 0;
   for(int i = 0; i < count; ++i) {
     const auto& q = problems[i];
     GProblem& P = pl.problems[i];
@@ -308,77 +225,41 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
     P.ldc = q.ldc;
     P.a_kmajor = q.transA == 0;
     P.b_kmajor = q.transB == 1;
-    P.config = choose_grouped_config(h, prec, q.m, q.n, q.k);
+    P.config = problem_config(pol, q.m, q.n);
     const int bm = host_bm(P.config), bn = host_bn(P.config);
     P.tiles_m = (int)ceil_div(q.m, bm);
     P.tiles_n = (int)ceil_div(q.n, bn);
     if(q.m == 0 || q.n == 0) P.tiles_m = P.tiles_n = 0;
     P.vec_a = f32_vec_ok(q.A, q.lda, P.a_kmajor, P.M, P.K);
     P.vec_b = f32_vec_ok(q.B, q.ldb, P.b_kmajor, P.N, P.K);
-    if(packed) {
-      // pack_chunk only takes 128-bit loads along k
-      P.vec_a = P.a_kmajor && P.vec_a;
-      P.vec_b = P.b_kmajor && P.vec_b;
-      P.kbs = (int)(round_up(q.k, 32) / 16);
-      packed_bytes += packed_sizes(P, nhl).first + packed_sizes(P, nhl).second;
-      P.packed_a = P.packed_b = nullptr;  // placed in bind_grouped
-      P.pack_a_flags = flags;
-      flags += P.tiles_m;
-      P.pack_b_flags = flags;
-      flags += P.tiles_n;
-    } else {
-      P.kbs = 0;
-      P.packed_a = P.packed_b = nullptr;
-      P.pack_a_flags = P.pack_b_flags = 0;
-    }
+    if(P.tiles_m * P.tiles_n > 0) pl.all_vec = pl.all_vec && P.vec_a && P.vec_b;
     pl.flops += 2.0 * q.m * q.n * q.k;
   }
-  pl.n_flags = flags;
 
-  // Items level by level: packs, then tiles (largest K first, a problem's
-  // tiles row by row).
+  // Items level by level, largest K first, a problem's tiles row by row.
   const int max_level = count ? *std::max_element(level.begin(), level.end()) : 0;
   for(int lv = 0; lv <= max_level; ++lv) {
     std::vector<int> at;
     for(int i = 0; i < count; ++i)
       if(level[i] == lv && pl.problems[i].tiles_m * pl.problems[i].tiles_n > 0) at.push_back(i);
-    if(packed)
-      for(int i : at) {
-        for(int p = 0; p < pl.problems[i].tiles_m; ++p) pl.items.push_back({ITEM_PACK_A, i, p, 0});
-        for(int p = 0; p < pl.problems[i].tiles_n; ++p) pl.items.push_back({ITEM_PACK_B, i, p, 0});
-      }
     std::stable_sort(at.begin(), at.end(), [&](int a, int b) { return pl.problems[a].K > pl.problems[b].K; });
     for(int i : at)
       for(int tm = 0; tm < pl.problems[i].tiles_m; ++tm)
-        for(int tn = 0; tn < pl.problems[i].tiles_n; ++tn) pl.items.push_back({ITEM_TILE, i, tm, tn});
+        for(int tn = 0; tn < pl.problems[i].tiles_n; ++tn) pl.items.push_back({i, tm, tn, 0});
   }
 
   pl.off_problems = 0;
   pl.off_items = round_up(sizeof(GProblem) * std::max(1, count), 256);
   pl.off_counters = pl.off_items + round_up(sizeof(GItem) * std::max<size_t>(1, pl.items.size()), 256);
-  pl.counter_ints = 2 + (size_t)count + (size_t)flags;
-  pl.off_packed = pl.off_counters + round_up(pl.counter_ints * sizeof(int), 256);
-  pl.packed_bytes = packed_bytes;
-  pl.total_bytes = pl.off_packed + packed_bytes;
+  pl.counter_ints = 2 + (size_t)count;
+  pl.total_bytes = pl.off_counters + round_up(pl.counter_ints * sizeof(int), 256);
   return MFX_SUCCESS;
 }
 
-// Places the packed operands in the device buffer `ws`, uploads the tables
-// (stream-ordered, from pinned staging memory owned by the handle) and fills
-// the kernel arguments.
+// Uploads the tables into the device buffer `ws` (stream-ordered, from
+// pinned staging memory owned by the handle) and fills the kernel arguments.
 static mfx_status bind_grouped(Handle* h, GroupedPlan& pl, void* ws, GroupedArgs& g, hipStream_t s) {
   char* base = static_cast<char*>(ws);
-  if(pl.prec != MFX_PREC_FP32) {
-    const int nhl = pl.prec == MFX_PREC_SPLIT_BF16 ? 2 : 1;
-    size_t off = pl.off_packed;
-    for(auto& P : pl.problems) {
-      const auto sz = packed_sizes(P, nhl);
-      P.packed_a = reinterpret_cast<u32x4*>(base + off);
-      off += sz.first;
-      P.packed_b = reinterpret_cast<u32x4*>(base + off);
-      off += sz.second;
-    }
-  }
   // The previous upload from the staging buffer must have completed.
   if(h->staging_ready) MFX_HIP(hipEventSynchronize(h->staging_ready));
   if(h->staging_bytes < pl.off_counters) {
@@ -401,7 +282,6 @@ static mfx_status bind_grouped(Handle* h, GroupedPlan& pl, void* ws, GroupedArgs
   g.head = counters;
   g.error = counters + 1;
   g.done = counters + 2;
-  g.pack_flags = counters + 2 + pl.problems.size();
   g.spin_limit = h->spin_limit;
   return MFX_SUCCESS;
 }
@@ -414,14 +294,8 @@ static mfx_status launch_grouped_kernels(Handle* h, const GroupedPlan& pl, const
   }));
   if(pl.items.empty()) return MFX_SUCCESS;
   const int grid = h->cu_count > 0 ? h->cu_count : 256;
-  switch(pl.prec) {
-    case MFX_PREC_FP32:
-      return timed_launch(h, "grouped_f32_mfma", s, [&] { grouped_f32_kernel<<<grid, 256, 0, s>>>(g); });
-    case MFX_PREC_SPLIT_BF16:
-      return timed_launch(h, "grouped_split_bf16_mfma", s, [&] { grouped_bf16x_kernel<2><<<grid, 256, 0, s>>>(g); });
-    default:
-      return timed_launch(h, "grouped_bf16_mfma", s, [&] { grouped_bf16x_kernel<1><<<grid, 256, 0, s>>>(g); });
-  }
+  if(pl.prec == MFX_PREC_FP32) return launch_grouped_f32(h, pl.big, pl.small, pl.all_vec, g, grid, s);
+  return launch_grouped_bf16(h, pl.prec == MFX_PREC_SPLIT_BF16 ? 2 : 1, pl.big, pl.small, pl.all_vec, g, grid, s);
 }
 
 mfx_status run_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* problems, int count,
@@ -477,7 +351,7 @@ using namespace mfx;
 
 struct mfx_graph_s {
   int device = 0;
-  void* mem = nullptr;  // tables, counters and packed panels of this graph
+  void* mem = nullptr;  // tables and counters of this graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int* error = nullptr;
@@ -533,7 +407,7 @@ mfx_status mfx_grouped_graph_create(mfx_handle handle, mfx_precision prec, const
   auto* gr = new(std::nothrow) mfx_graph_s;
   if(!gr) return MFX_ERROR_ALLOC;
   gr->device = h->device;
-  // The graph owns its tables, counters and packed panels, so later calls on
+  // The graph owns its tables and counters, so later calls on
   // the handle (which may regrow the handle's workspace) cannot move them.
   hipError_t ae = hipMalloc(&gr->mem, std::max<size_t>(pl.total_bytes, 256));
   if(ae != hipSuccess) {

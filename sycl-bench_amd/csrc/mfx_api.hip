@@ -172,24 +172,11 @@ int choose_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k) {
   return heuristic_config(h, p, m, n, k);
 }
 
-// Grouped launches fill the machine with many problems at once, so each
-// problem takes the largest tile whose padded area exceeds the problem's by at
-// most `grouped_waste` (tuned entries under the "g" prefix take precedence).
-int choose_grouped_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k) {
-  if(h->forced_config[p] >= 0) return h->forced_config[p];
-  {
-    std::lock_guard<std::mutex> lock(g_tune_mutex);
-    auto it = g_tuned.find("g" + tune_key(p, m, n, k));
-    if(it != g_tuned.end()) return it->second;
-  }
-  const int order[4] = {0, 3, 1, 2};
-  for(int oi = 0; oi < 4; ++oi) {
-    int bm, bn, bk, w;
-    if(!config_info(p, order[oi], &bm, &bn, &bk, &w)) continue;
-    const double padded = (double)round_up(m, bm) * (double)round_up(n, bn);
-    if(padded <= h->grouped_waste * (double)m * (double)n) return order[oi];
-  }
-  return 2;
+// Tuned configuration (or grouped tile policy) recorded under `key`, -1 if none.
+int tuned_lookup(const std::string& key) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  auto it = g_tuned.find(key);
+  return it == g_tuned.end() ? -1 : it->second;
 }
 
 // Square-tile choice for the triangular schedules.

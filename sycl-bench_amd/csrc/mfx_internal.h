@@ -62,7 +62,7 @@ struct Handle {
   // MFX_SPIN_LIMIT overrides), pinned staging for the work tables and the
   // event that marks their upload, and the error flag of the last launch.
   int spin_limit = 1 << 22;
-  double grouped_waste = 1.34;  // padded-area bound of the grouped tile choice
+  int grouped_policy = -1;  // forced grouped tile policy (-1: tuned / estimated)
   void* staging = nullptr;
   size_t staging_bytes = 0;
   hipEvent_t staging_ready = nullptr;

@@ -30,6 +30,35 @@ __device__ __forceinline__ uint32_t pack_bf16_pair(uint16_t a, uint16_t b) {
   return (uint32_t)a | ((uint32_t)b << 16);
 }
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// Two floats to a packed pair of BF16, round to nearest even, in one
+// instruction (v_cvt_pk_bf16_f32); equals bf16_rne on every non-NaN input.
+__device__ __forceinline__ uint32_t cvt_pk_bf16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+
+// Splits 8 floats into their BF16 hi parts (and, for NHL == 2, the BF16 lo
+// parts of the residuals), packed as one 16-byte chunk each.
+template <int NHL>
+__device__ __forceinline__ void split_chunk(const float (&v)[8], u32x4& h, u32x4& l) {
+  uint32_t hw[4], lw[4];
+#pragma unroll
+  for(int p = 0; p < 4; ++p) {
+    hw[p] = cvt_pk_bf16(v[2 * p], v[2 * p + 1]);
+    if(NHL == 2) {
+      const float h0 = __builtin_bit_cast(float, hw[p] << 16);
+      const float h1 = __builtin_bit_cast(float, hw[p] & 0xffff0000u);
+      lw[p] = cvt_pk_bf16(v[2 * p] - h0, v[2 * p + 1] - h1);
+    } else {
+      lw[p] = 0;
+    }
+  }
+  h = (u32x4){hw[0], hw[1], hw[2], hw[3]};
+  l = (u32x4){lw[0], lw[1], lw[2], lw[3]};
+}
+
 // Source view of one operand for packing.
 struct PackSrc {
   const float* src;
@@ -64,27 +93,11 @@ __device__ __forceinline__ void pack_chunk(const PackSrc& s, int blk, int kb, in
 #pragma unroll
     for(int e = 0; e < 8; ++e) v[e] = (r < s.rows && k0 + e < s.K) ? s.src[(int64_t)(k0 + e) * s.ld + r] : 0.0f;
   }
-  uint16_t hi[8], lo[8];
-#pragma unroll
-  for(int e = 0; e < 8; ++e) {
-    hi[e] = bf16_rne(v[e]);
-    lo[e] = bf16_rne(v[e] - bf16_to_f32(hi[e]));
-  }
   const int64_t base = ((int64_t)blk * kbs + kb) * NHL * 64 + lane;
-  u32x4 h;
-  h.x = pack_bf16_pair(hi[0], hi[1]);
-  h.y = pack_bf16_pair(hi[2], hi[3]);
-  h.z = pack_bf16_pair(hi[4], hi[5]);
-  h.w = pack_bf16_pair(hi[6], hi[7]);
+  u32x4 h, l;
+  split_chunk<NHL>(v, h, l);
   dst[base] = h;
-  if(NHL == 2) {
-    u32x4 l;
-    l.x = pack_bf16_pair(lo[0], lo[1]);
-    l.y = pack_bf16_pair(lo[2], lo[3]);
-    l.z = pack_bf16_pair(lo[4], lo[5]);
-    l.w = pack_bf16_pair(lo[6], lo[7]);
-    dst[base + 64] = l;
-  }
+  if(NHL == 2) dst[base + 64] = l;
 }
 
 template <int BM, int BN, int WM, int WN, int NHL>
@@ -109,9 +122,70 @@ struct BfTileArgs {
 
 __device__ __forceinline__ bf16x8 as_bf16x8(u32x4 u) { return __builtin_bit_cast(bf16x8, u); }
 
-// Computes tile (tm, tn) and writes it through t.ep.  `lds` holds
-// BfCfg::SMEM_CHUNKS chunks; the caller separates two calls that reuse it
-// with a barrier.
+// One 16-byte chunk per lane from global memory straight into LDS
+// (global_load_lds_dwordx4): lane l's chunk lands at lds_wave_base + l.  The
+// LDS base must be the same for the whole wave.
+__device__ __forceinline__ void lds_dma16(const u32x4* src, u32x4* lds_wave_base) {
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+      (__attribute__((address_space(3))) void*)lds_wave_base, 16, 0, 0);
+}
+
+// Waits for this wave's outstanding memory operations, LDS transfers included.
+__device__ __forceinline__ void lds_dma_wait() { __builtin_amdgcn_s_waitcnt(0); }
+
+// MFMAs of one LDS stage: acc[i][j] += A-fragments x B-fragments over the
+// stage's KB_PER_STAGE k blocks; sa / sb are the stage's A / B chunks.
+template <int FM, int FN, int NHL>
+__device__ __forceinline__ void bf16_stage_mma(floatx16 (&acc)[FM][FN], const u32x4* sa, const u32x4* sb, int wm,
+    int wn, int lane) {
+#pragma unroll
+  for(int kb = 0; kb < KB_PER_STAGE; ++kb) {
+    bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+    for(int i = 0; i < FM; ++i) {
+      const int blk = wm * FM + i;
+      ah[i] = as_bf16x8(sa[(blk * KB_PER_STAGE + kb) * NHL * 64 + lane]);
+      if(NHL == 2) al[i] = as_bf16x8(sa[((blk * KB_PER_STAGE + kb) * NHL + 1) * 64 + lane]);
+    }
+#pragma unroll
+    for(int j = 0; j < FN; ++j) {
+      const int blk = wn * FN + j;
+      bh[j] = as_bf16x8(sb[(blk * KB_PER_STAGE + kb) * NHL * 64 + lane]);
+      if(NHL == 2) bl[j] = as_bf16x8(sb[((blk * KB_PER_STAGE + kb) * NHL + 1) * 64 + lane]);
+    }
+    // Product-major order: consecutive MFMAs write different accumulators,
+    // so none waits on the previous one's result.  The small cross terms go
+    // in first so they are added while the accumulator is smallest.
+    if(NHL == 2) {
+#pragma unroll
+      for(int i = 0; i < FM; ++i)
+#pragma unroll
+        for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for(int i = 0; i < FM; ++i)
+#pragma unroll
+        for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for(int i = 0; i < FM; ++i)
+#pragma unroll
+      for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+  }
+}
+
+template <int FM, int FN>
+__device__ __forceinline__ void zero_acc(floatx16 (&acc)[FM][FN]) {
+#pragma unroll
+  for(int i = 0; i < FM; ++i)
+#pragma unroll
+    for(int j = 0; j < FN; ++j)
+#pragma unroll
+      for(int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
+}
+
+// Computes tile (tm, tn) from packed operands and writes it through t.ep.
+// `lds` holds BfCfg::SMEM_CHUNKS chunks; the caller separates two calls that
+// reuse it with a barrier.
 template <int BM, int BN, int WM, int WN, int NHL>
 __device__ __forceinline__ void bf16_tile(const BfTileArgs& t, int tm, int tn, u32x4* lds) {
   using Cfg = BfCfg<BM, BN, WM, WN, NHL>;
@@ -121,12 +195,7 @@ __device__ __forceinline__ void bf16_tile(const BfTileArgs& t, int tm, int tn, u
   const int wm = wid / Cfg::NWN, wn = wid % Cfg::NWN;
 
   floatx16 acc[FM][FN];
-#pragma unroll
-  for(int i = 0; i < FM; ++i)
-#pragma unroll
-    for(int j = 0; j < FN; ++j)
-#pragma unroll
-      for(int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
+  zero_acc(acc);
 
   // Chunk c of a stage: block c / RUN of the tile, then the contiguous
   // (kb, hl, lane) run of that block for this stage.
@@ -134,74 +203,179 @@ __device__ __forceinline__ void bf16_tile(const BfTileArgs& t, int tm, int tn, u
   const int KT = t.kbs / KB_PER_STAGE;
   const int steps = KT * t.nprod;
   const int64_t blk_stride = (int64_t)t.kbs * NHL * 64;
-  u32x4 ra[Cfg::PER_A], rb[Cfg::PER_B];
-
-  auto issue = [&](int s) __attribute__((always_inline)) {
+  // Global -> LDS without registers (global_load_lds_dwordx4): for one wave
+  // and one i, the 64 lanes' chunks c = wave*64 + i*NT + lane are 64
+  // consecutive chunks both in the packed operand (RUN is a multiple of 64)
+  // and in the LDS stage, so each instruction moves one contiguous 1 KiB.
+  const int wave_base = tid - lane;
+  auto issue = [&](int s, int buf) __attribute__((always_inline)) {
     const int prod = s / KT, kt = s - prod * KT;
     const u32x4* A = prod ? t.PB : t.PA;
     const u32x4* B = prod ? t.PA : t.PB;
     const int64_t koff = (int64_t)kt * RUN;
+    u32x4* sa = lds + buf * (SA + SB);
+    u32x4* sb = sa + SA;
 #pragma unroll
     for(int i = 0; i < Cfg::PER_A; ++i) {
       const int c = tid + i * NT;
       const int blk = c / RUN, run = c - blk * RUN;
-      ra[i] = A[(int64_t)(tm * (BM / 32) + blk) * blk_stride + koff + run];
+      lds_dma16(A + (int64_t)(tm * (BM / 32) + blk) * blk_stride + koff + run, sa + wave_base + i * NT);
     }
 #pragma unroll
     for(int i = 0; i < Cfg::PER_B; ++i) {
       const int c = tid + i * NT;
       const int blk = c / RUN, run = c - blk * RUN;
-      rb[i] = B[(int64_t)(tn * (BN / 32) + blk) * blk_stride + koff + run];
+      lds_dma16(B + (int64_t)(tn * (BN / 32) + blk) * blk_stride + koff + run, sb + wave_base + i * NT);
     }
-  };
-  auto commit = [&](int buf) __attribute__((always_inline)) {
-    u32x4* sa = lds + buf * (SA + SB);
-    u32x4* sb = sa + SA;
-#pragma unroll
-    for(int i = 0; i < Cfg::PER_A; ++i) sa[tid + i * NT] = ra[i];
-#pragma unroll
-    for(int i = 0; i < Cfg::PER_B; ++i) sb[tid + i * NT] = rb[i];
   };
   auto compute = [&](int cur) __attribute__((always_inline)) {
     const u32x4* sa = lds + cur * (SA + SB);
-    const u32x4* sb = sa + SA;
-#pragma unroll
-    for(int kb = 0; kb < KB_PER_STAGE; ++kb) {
-      bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
-#pragma unroll
-      for(int i = 0; i < FM; ++i) {
-        const int blk = wm * FM + i;
-        ah[i] = as_bf16x8(sa[(blk * KB_PER_STAGE + kb) * NHL * 64 + lane]);
-        if(NHL == 2) al[i] = as_bf16x8(sa[((blk * KB_PER_STAGE + kb) * NHL + 1) * 64 + lane]);
-      }
-#pragma unroll
-      for(int j = 0; j < FN; ++j) {
-        const int blk = wn * FN + j;
-        bh[j] = as_bf16x8(sb[(blk * KB_PER_STAGE + kb) * NHL * 64 + lane]);
-        if(NHL == 2) bl[j] = as_bf16x8(sb[((blk * KB_PER_STAGE + kb) * NHL + 1) * 64 + lane]);
-      }
-      // Product-major order: consecutive MFMAs write different accumulators,
-      // so none waits on the previous one's result.  The small cross terms go
-      // in first so they are added while the accumulator is smallest.
-      if(NHL == 2) {
-#pragma unroll
-        for(int i = 0; i < FM; ++i)
-#pragma unroll
-          for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-        for(int i = 0; i < FM; ++i)
-#pragma unroll
-          for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-      }
-#pragma unroll
-      for(int i = 0; i < FM; ++i)
-#pragma unroll
-        for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-    }
+    bf16_stage_mma<FM, FN, NHL>(acc, sa, sa + SA, wm, wn, lane);
   };
 
-  // Steady state without branches around the prefetch, so the registers
-  // holding stage s+1 stay live across the MFMAs of stage s (no early wait).
+  // Stage s+1 streams into the other buffer while stage s is contracted; the
+  // buffer it overwrites was last read before the previous barrier.
+  if(steps > 0) issue(0, 0);
+  lds_dma_wait();
+  __syncthreads();
+  for(int s = 0; s + 1 < steps; ++s) {
+    issue(s + 1, (s + 1) & 1);
+    compute(s & 1);
+    lds_dma_wait();
+    __syncthreads();
+  }
+  if(steps > 0) compute((steps - 1) & 1);
+
+  static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
+    constexpr int i = decltype(ij)::value / FN, j = decltype(ij)::value % FN;
+    store_block32(t.ep, acc[i][j], tm * BM + wm * WM + i * 32, tn * BN + wn * WN + j * 32, lane);
+  });
+}
+
+// Stages one ROWS x 32 panel of an FP32 operand (rows row0.., k k0..) through
+// registers, splitting it on the way into the fragment-order LDS stage that
+// bf16_stage_mma reads.  Work unit = 8 consecutive k of one row (one chunk);
+// each of the NT threads owns UPT = ROWS*4/NT units:
+//   KMAJOR (element (r, k) at P[r*ld + k]): row tid % ROWS, UPT consecutive
+//       8-k segments -- one contiguous 32*UPT-byte read per thread;
+//   else (element (r, k) at P[k*ld + r]): UPT consecutive rows and one 8-k
+//       segment -- eight UPT-wide reads along the rows, consecutive threads
+//       on consecutive rows.
+// VEC: 128-bit (or UPT-wide) loads, which need aligned rows and a contiguous
+// extent that is a multiple of 4; otherwise bounds-checked scalar loads.
+template <int ROWS, int NT, int NHL, bool KMAJOR, bool VEC>
+struct SplitLoader {
+  static constexpr int UPT = ROWS * 4 / NT;
+  static_assert(UPT == 1 || UPT == 2 || UPT == 4, "unsupported panel shape");
+  static_assert(ROWS / UPT == 64 || KMAJOR, "one wave per 8-k segment");
+  float v[UPT][8];
+
+  __device__ __forceinline__ void load(const float* P, int64_t ld, int rows, int K, int row0, int k0, int tid) {
+    if constexpr(KMAJOR) {
+      const int r = tid % ROWS, seg0 = tid / ROWS * UPT;
+      const int gr = row0 + r;
+      const float* src = P + (int64_t)gr * ld;
+#pragma unroll
+      for(int u = 0; u < UPT; ++u) {
+        const int gk = k0 + (seg0 + u) * 8;
+        if constexpr(VEC) {
+          float4 x = make_float4(0.f, 0.f, 0.f, 0.f), y = x;
+          if(gr < rows && gk < K) x = *reinterpret_cast<const float4*>(src + gk);
+          if(gr < rows && gk + 4 < K) y = *reinterpret_cast<const float4*>(src + gk + 4);
+          v[u][0] = x.x; v[u][1] = x.y; v[u][2] = x.z; v[u][3] = x.w;
+          v[u][4] = y.x; v[u][5] = y.y; v[u][6] = y.z; v[u][7] = y.w;
+        } else {
+#pragma unroll
+          for(int e = 0; e < 8; ++e) v[u][e] = (gr < rows && gk + e < K) ? src[gk + e] : 0.0f;
+        }
+      }
+    } else {
+      const int r0 = (tid % 64) * UPT, seg = tid / 64;
+      const int gr = row0 + r0;
+#pragma unroll
+      for(int e = 0; e < 8; ++e) {
+        const int gk = k0 + seg * 8 + e;
+        const float* src = P + (int64_t)gk * ld + gr;
+        if constexpr(VEC && UPT == 4) {
+          float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+          if(gk < K && gr < rows) x = *reinterpret_cast<const float4*>(src);
+          v[0][e] = x.x; v[1][e] = x.y; v[2][e] = x.z; v[3][e] = x.w;
+        } else if constexpr(VEC && UPT == 2) {
+          float2 x = make_float2(0.f, 0.f);
+          if(gk < K && gr < rows) x = *reinterpret_cast<const float2*>(src);
+          v[0][e] = x.x; v[1][e] = x.y;
+        } else {
+#pragma unroll
+          for(int u = 0; u < UPT; ++u) v[u][e] = (gk < K && gr + u < rows) ? src[u] : 0.0f;
+        }
+      }
+    }
+  }
+
+  // Splits the registers and writes them into the stage `st` (chunks
+  // [blk][kb][hl][lane] for this operand).
+  __device__ __forceinline__ void commit(u32x4* st, int tid) const {
+#pragma unroll
+    for(int u = 0; u < UPT; ++u) {
+      int r, seg;
+      if constexpr(KMAJOR) {
+        r = tid % ROWS;
+        seg = tid / ROWS * UPT + u;
+      } else {
+        r = (tid % 64) * UPT + u;
+        seg = tid / 64;
+      }
+      const int blk = r >> 5, kb = seg >> 1, lane = (r & 31) + 32 * (seg & 1);
+      u32x4 h, l;
+      split_chunk<NHL>(v[u], h, l);
+      u32x4* c = st + (blk * KB_PER_STAGE + kb) * NHL * 64 + lane;
+      c[0] = h;
+      if(NHL == 2) c[64] = l;
+    }
+  }
+};
+
+// Computes tile (tm, tn) of `op` straight from its FP32 operands, splitting
+// them into BF16 while staging (no packed copies), and writes it through
+// op.ep.  Same LDS size and MFMA sequence as bf16_tile, so the two give
+// bit-identical results.
+template <int BM, int BN, int WM, int WN, int NHL, bool AK, bool BKM, bool VEC>
+__device__ __forceinline__ void bf16_tile_direct(const DenseOp& op, int tm, int tn, u32x4* lds) {
+  using Cfg = BfCfg<BM, BN, WM, WN, NHL>;
+  constexpr int NT = Cfg::NT, FM = Cfg::FM, FN = Cfg::FN;
+  constexpr int SA = Cfg::CH_A, SB = Cfg::CH_B;
+  constexpr int BK = KB_PER_STAGE * 16;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm = wid / Cfg::NWN, wn = wid % Cfg::NWN;
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  floatx16 acc[FM][FN];
+  zero_acc(acc);
+
+  SplitLoader<BM, NT, NHL, AK, VEC> la;
+  SplitLoader<BN, NT, NHL, BKM, VEC> lb;
+  const int KT = (op.K + BK - 1) / BK;
+  const int steps = KT * op.nprod;
+  auto issue = [&](int s) __attribute__((always_inline)) {
+    const int prod = s / KT, kt = s - prod * KT;
+    const float* pa = prod ? op.B : op.A;
+    const float* pb = prod ? op.A : op.B;
+    const int64_t lda = prod ? op.ldb : op.lda, ldb = prod ? op.lda : op.ldb;
+    la.load(pa, lda, op.M, op.K, m0, kt * BK, tid);
+    lb.load(pb, ldb, op.N, op.K, n0, kt * BK, tid);
+  };
+  auto commit = [&](int buf) __attribute__((always_inline)) {
+    u32x4* sa = lds + buf * (SA + SB);
+    la.commit(sa, tid);
+    lb.commit(sa + SA, tid);
+  };
+  auto compute = [&](int cur) __attribute__((always_inline)) {
+    const u32x4* sa = lds + cur * (SA + SB);
+    bf16_stage_mma<FM, FN, NHL>(acc, sa, sa + SA, wm, wn, lane);
+  };
+
+  // As in f32_tile: loads of step s+1 are in flight during the MFMAs of step
+  // s and land in the other buffer afterwards; one barrier per step.
   if(steps > 0) {
     issue(0);
     commit(0);
@@ -217,7 +391,7 @@ __device__ __forceinline__ void bf16_tile(const BfTileArgs& t, int tm, int tn, u
 
   static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
     constexpr int i = decltype(ij)::value / FN, j = decltype(ij)::value % FN;
-    store_block32(t.ep, acc[i][j], tm * BM + wm * WM + i * 32, tn * BN + wn * WN + j * 32, lane);
+    store_block32(op.ep, acc[i][j], m0 + wm * WM + i * 32, n0 + wn * WN + j * 32, lane);
   });
 }
 

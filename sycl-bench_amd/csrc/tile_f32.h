@@ -33,8 +33,10 @@ struct PanelLoader {
   static_assert(NV % NT == 0, "panel must split evenly over threads");
   float4 v[PER];
 
-  __device__ __forceinline__ void load(const float* P, int64_t ld, int rows, int K, int row0, int k0, int tid,
-      bool vec) {
+  // VEC: 128-bit loads (compile-time, so the steady-state loop has no
+  // per-load branches); otherwise four bounds-checked scalar loads.
+  template <bool VEC>
+  __device__ __forceinline__ void load(const float* P, int64_t ld, int rows, int K, int row0, int k0, int tid) {
 #pragma unroll
     for(int i = 0; i < PER; ++i) {
       const int idx = tid + i * NT;
@@ -51,7 +53,7 @@ struct PanelLoader {
       if(KMAJOR) {
         const float* src = P + (int64_t)gr * ld + gk;
         if(gr < rows) {
-          if(vec) {
+          if constexpr(VEC) {
             if(gk < K) x = *reinterpret_cast<const float4*>(src);
           } else {
             if(gk + 0 < K) x.x = src[0];
@@ -63,7 +65,7 @@ struct PanelLoader {
       } else {
         const float* src = P + (int64_t)gk * ld + gr;
         if(gk < K) {
-          if(vec) {
+          if constexpr(VEC) {
             if(gr < rows) x = *reinterpret_cast<const float4*>(src);
           } else {
             if(gr + 0 < rows) x.x = src[0];
@@ -107,8 +109,8 @@ inline bool f32_vec_ok(const float* P, int64_t ld, bool kmajor, int rows, int K)
 // Computes tile (tm, tn) of `op` (all nprod products) and writes it through
 // op.ep.  `smem` holds F32Cfg::SMEM_FLOATS floats; the caller separates two
 // calls that reuse it with a barrier.
-template <int BM, int BN, int BK, int WM, int WN, bool AK, bool BKM>
-__device__ __forceinline__ void f32_tile(const DenseOp& op, bool vec_a, bool vec_b, int tm, int tn, float* smem) {
+template <int BM, int BN, int BK, int WM, int WN, bool AK, bool BKM, bool VEC>
+__device__ __forceinline__ void f32_tile(const DenseOp& op, int tm, int tn, float* smem) {
   using Cfg = F32Cfg<BM, BN, BK, WM, WN>;
   constexpr int NT = Cfg::NT, FM = Cfg::FM, FN = Cfg::FN;
   typedef float PanelA[BK][Cfg::LDSA];
@@ -138,9 +140,8 @@ __device__ __forceinline__ void f32_tile(const DenseOp& op, bool vec_a, bool vec
     const float* pa = prod ? op.B : op.A;
     const float* pb = prod ? op.A : op.B;
     const int64_t lda = prod ? op.ldb : op.lda, ldb = prod ? op.lda : op.ldb;
-    const bool va = prod ? vec_b : vec_a, vb = prod ? vec_a : vec_b;
-    la.load(pa, lda, op.M, op.K, m0, kt * BK, tid, va);
-    lb.load(pb, ldb, op.N, op.K, n0, kt * BK, tid, vb);
+    la.template load<VEC>(pa, lda, op.M, op.K, m0, kt * BK, tid);
+    lb.template load<VEC>(pb, ldb, op.N, op.K, n0, kt * BK, tid);
   };
   auto commit = [&](int buf) __attribute__((always_inline)) {
     la.template store<Cfg::LDSA>(As[buf], tid);

@@ -61,7 +61,10 @@ SIGNATURES = {
     "mfx_graph_destroy": (_ST, [_V]),
     "mfx_grouped_status": (_ST, [_V, _V]),
     "mfx_autotune_grouped": (_ST, [_V, _ST, ctypes.POINTER(Problem), _I32, ctypes.POINTER(Dep), _I32,
-                                   ctypes.POINTER(_D), _V]),
+                                   ctypes.POINTER(_I32), _V]),
+    "mfx_grouped_policy_count": (_I32, []),
+    "mfx_grouped_policy_info": (_ST, [_I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_D)]),
+    "mfx_force_grouped_policy": (_ST, [_V, _I32]),
     "mfx_gemm_f64": (_ST, [_V, _ST, _ST, _I64, _I64, _I64, _D, _V, _I64, _V, _I64, _D, _V, _I64, _V]),
     "mfx_convert_f32_f64": (_ST, [_V, _I64, _I64, _V, _I64, _V, _I64, _V]),
     "mfx_rel_frobenius": (_ST, [_V, _ST, _I64, _I64, _V, _I64, _V, _I64, ctypes.POINTER(_D), _V]),
@@ -220,6 +223,9 @@ class Handle:
     def force_config(self, prec, config):
         _check(self.lib.mfx_force_config(self.h, prec_id(prec), config), "mfx_force_config")
 
+    def force_grouped_policy(self, policy):
+        _check(self.lib.mfx_force_grouped_policy(self.h, policy), "mfx_force_grouped_policy")
+
     def timing_begin(self):
         _check(self.lib.mfx_timing_begin(self.h), "mfx_timing_begin")
 
@@ -243,10 +249,10 @@ class Handle:
 
     def autotune_grouped(self, problems, deps=(), prec="split-bf16", stream=None):
         arr, darr = _problem_arrays(problems, deps)
-        w = ctypes.c_double()
+        best = ctypes.c_int32()
         _check(self.lib.mfx_autotune_grouped(self.h, prec_id(prec), arr, len(problems), darr, len(deps),
-                                             ctypes.byref(w), _stream(stream)), "mfx_autotune_grouped")
-        return w.value
+                                             ctypes.byref(best), _stream(stream)), "mfx_autotune_grouped")
+        return best.value
 
     def chain(self, mats, scratch, out, prec="split-bf16", stream=None):
         n = out.shape[0]
@@ -312,3 +318,15 @@ def config_info(prec, c):
     v = [ctypes.c_int32() for _ in range(4)]
     _check(load().mfx_config_info(prec_id(prec), c, *[ctypes.byref(x) for x in v]), "mfx_config_info")
     return tuple(x.value for x in v)
+
+
+def grouped_policy_count():
+    return load().mfx_grouped_policy_count()
+
+
+def grouped_policy_info(policy):
+    """(large config, small config, waste bound) of a grouped tile policy."""
+    big, small, bound = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+    _check(load().mfx_grouped_policy_info(policy, ctypes.byref(big), ctypes.byref(small), ctypes.byref(bound)),
+           "mfx_grouped_policy_info")
+    return big.value, small.value, bound.value

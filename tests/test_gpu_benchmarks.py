@@ -83,6 +83,15 @@ def test_matmulchain_uniform_chain_of_eight():
     assert float(res["rel-frobenius-error"]) < 1e-5 * 7 ** 0.5
 
 
+@pytest.mark.parametrize("init", ["polybench", "uniform"])
+def test_matmulchain_grouped_launch(init):
+    r, res = bench("matmulchain", "--size=512", "--num-runs=2", f"--init={init}", "--chain-length=8", "--grouped",
+                   "--precision=split-bf16", "--queue-profiling")
+    assert r.returncode == 0, r.stderr
+    assert res["Verification"] == "PASS"
+    assert res["_names"][0].endswith("_len8_grouped")
+
+
 def test_csv_output_header(tmp_path):
     csv = tmp_path / "out.csv"
     r, _ = bench("gemm", "--size=256", "--num-runs=2", f"--output={csv}")

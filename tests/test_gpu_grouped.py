@@ -182,7 +182,7 @@ def test_autotune_single_and_table_roundtrip(handle, mfx, tmp_path):
     assert lib.mfx_autotune_save(path.encode()) == 0
     assert lib.mfx_autotune_load(path.encode()) == 0
     text = open(path).read()
-    assert "mfx-autotune-1" in text and '"1:9:9:9": %d' % best in text
+    assert "mfx-autotune-2" in text and '"1:9:9:9": %d' % best in text
     # a tuned gemm still matches FP64
     A = torch.empty((512, 512), device=DEV)
     B = torch.empty((512, 512), device=DEV)
@@ -198,8 +198,65 @@ def test_autotune_grouped(handle, mfx):
     keep = rand_problems(mfx, rng, 30, lo=64, hi=1024, transposes=False)
     fill(handle, keep, 8)
     probs = [mfx.problem(A, B, C) for A, B, C, ta, tb, _ in keep]
-    w = handle.autotune_grouped(probs, prec="split-bf16")
-    assert w >= 1.0
+    p = handle.autotune_grouped(probs, prec="split-bf16")
+    assert 0 <= p < mfx.grouped_policy_count()
     handle.grouped(probs, prec="split-bf16")
     for A, B, C, ta, tb, _ in keep:
         assert rel(C, A.double() @ B.double()) < 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp32", "split-bf16", "bf16"])
+def test_every_tile_policy(handle, mfx, prec):
+    """Each grouped tile policy (pairs of tile configurations) against FP64,
+    and bit-equal to single-problem launches on the tile each problem got."""
+    rng = np.random.default_rng(11)
+    keep = rand_problems(mfx, rng, 24, lo=16, hi=700)
+    try:
+        for pol in range(mfx.grouped_policy_count()):
+            big, small, bound = mfx.grouped_policy_info(pol)
+            fill(handle, keep, 6)
+            C0 = [k[2].clone() for k in keep]
+            probs = [mfx.problem(A, B, C, alpha=0.5, beta=2.0, transA=ta, transB=tb) for A, B, C, ta, tb, _ in keep]
+            handle.force_grouped_policy(pol)
+            handle.grouped(probs, prec=prec)
+            handle.grouped_status()
+            for (A, B, C, ta, tb, _), c0 in zip(keep, C0):
+                err = rel(C, fp64(A, B, c0, 0.5, 2.0, ta, tb))
+                assert err < TOL[prec], (pol, err)
+            if big != small:
+                got = [k[2].clone() for k in keep]
+                fill(handle, keep, 6)
+                for (A, B, C, ta, tb, _), g in zip(keep, got):
+                    m, n = C.shape
+                    bm, bn = mfx.config_info(prec, big)[:2]
+                    cfg = small if -(-m // bm) * bm * -(-n // bn) * bn > bound * m * n else big
+                    handle.force_config(prec, cfg)
+                    handle.gemm(A, B, C, alpha=0.5, beta=2.0, prec=prec, transA=ta, transB=tb)
+                    handle.force_config(prec, -1)
+                    assert torch.equal(C, g), (pol, cfg, m, n)
+    finally:
+        handle.force_grouped_policy(-1)
+        handle.force_config(prec, -1)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "split-bf16", "bf16"])
+def test_grouped_odd_shapes_scalar_loads(handle, mfx, prec):
+    """Extents that are not multiples of 4 force the bounds-checked scalar
+    loads; K = 0 leaves beta * C."""
+    shapes = [(33, 17, 45, False, False), (130, 257, 7, True, False), (5, 300, 129, False, True),
+              (257, 255, 253, True, True), (64, 48, 0, False, False)]
+    keep, probs = [], []
+    for i, (m, n, k, ta, tb) in enumerate(shapes):
+        A = torch.empty((k, m) if ta else (m, k), device=DEV)
+        B = torch.empty((n, k) if tb else (k, n), device=DEV)
+        C = torch.empty((m, n), device=DEV)
+        for j, x in enumerate((A, B, C)):
+            if x.numel():
+                handle.fill_uniform(x, 9, 3 * i + j)
+        keep.append((A, B, C, ta, tb, C.clone()))
+        probs.append(mfx.problem(A, B, C, alpha=1.25, beta=-0.5, transA=ta, transB=tb))
+    handle.grouped(probs, prec=prec)
+    handle.grouped_status()
+    for A, B, C, ta, tb, c0 in keep:
+        want = fp64(A, B, c0, 1.25, -0.5, ta, tb)
+        assert rel(C, want) < TOL[prec], (C.shape, A.shape)
