@@ -110,7 +110,7 @@ int choose_grouped_policy(Handle* h, mfx_precision prec, const mfx_gemm_problem*
   if(h->grouped_policy >= 0) return h->grouped_policy;
   int forced = h->forced_config[prec];
   if(forced >= 0) {
-    if(forced == 0) forced = 3;  // no grouped 256x256 kernel
+    if(forced == 0 || forced == 4) forced = 3;  // no grouped 256x256 kernel
     for(int i = 0; i < kPolicyCount; ++i)
       if(kPolicies[i].big == forced && kPolicies[i].small == forced) return i;
   }

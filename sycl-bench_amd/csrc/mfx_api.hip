@@ -148,9 +148,10 @@ int heuristic_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k
   const int cus = h->cu_count > 0 ? h->cu_count : 256;
   const int count = config_count(p);
   int best = count > 0 ? count - 1 : 0;
-  // configs 0 (256x256), 3 (256x128), 1 (128x128), 2 (64x64) in order of size
-  const int order[4] = {0, 3, 1, 2};
-  for(int oi = 0; oi < 4; ++oi) {
+  // configs 4 (256x256, eight waves), 0 (256x256), 3 (256x128), 1 (128x128),
+  // 2 (64x64) in order of size; precisions without config 4 skip it
+  const int order[5] = {4, 0, 3, 1, 2};
+  for(int oi = 0; oi < 5; ++oi) {
     const int c = order[oi];
     int bm, bn, bk, w;
     if(c >= count || !config_info(p, c, &bm, &bn, &bk, &w)) continue;

@@ -133,44 +133,49 @@ __device__ __forceinline__ void lds_dma16(const u32x4* src, u32x4* lds_wave_base
 // Waits for this wave's outstanding memory operations, LDS transfers included.
 __device__ __forceinline__ void lds_dma_wait() { __builtin_amdgcn_s_waitcnt(0); }
 
-// MFMAs of one LDS stage: acc[i][j] += A-fragments x B-fragments over the
-// stage's KB_PER_STAGE k blocks; sa / sb are the stage's A / B chunks.
+// MFMAs of one 16-wide k block `kb` of an LDS stage: acc[i][j] +=
+// A-fragments x B-fragments; sa / sb are the stage's A / B chunks.
+template <int FM, int FN, int NHL>
+__device__ __forceinline__ void bf16_kb_mma(floatx16 (&acc)[FM][FN], const u32x4* sa, const u32x4* sb, int wm, int wn,
+    int lane, int kb) {
+  bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+  for(int i = 0; i < FM; ++i) {
+    const int blk = wm * FM + i;
+    ah[i] = as_bf16x8(sa[(blk * KB_PER_STAGE + kb) * NHL * 64 + lane]);
+    if(NHL == 2) al[i] = as_bf16x8(sa[((blk * KB_PER_STAGE + kb) * NHL + 1) * 64 + lane]);
+  }
+#pragma unroll
+  for(int j = 0; j < FN; ++j) {
+    const int blk = wn * FN + j;
+    bh[j] = as_bf16x8(sb[(blk * KB_PER_STAGE + kb) * NHL * 64 + lane]);
+    if(NHL == 2) bl[j] = as_bf16x8(sb[((blk * KB_PER_STAGE + kb) * NHL + 1) * 64 + lane]);
+  }
+  // Product-major order: consecutive MFMAs write different accumulators, so
+  // none waits on the previous one's result.  The small cross terms go in
+  // first so they are added while the accumulator is smallest.
+  if(NHL == 2) {
+#pragma unroll
+    for(int i = 0; i < FM; ++i)
+#pragma unroll
+      for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for(int i = 0; i < FM; ++i)
+#pragma unroll
+      for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+  }
+#pragma unroll
+  for(int i = 0; i < FM; ++i)
+#pragma unroll
+    for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+}
+
+// MFMAs of a whole LDS stage (KB_PER_STAGE k blocks).
 template <int FM, int FN, int NHL>
 __device__ __forceinline__ void bf16_stage_mma(floatx16 (&acc)[FM][FN], const u32x4* sa, const u32x4* sb, int wm,
     int wn, int lane) {
 #pragma unroll
-  for(int kb = 0; kb < KB_PER_STAGE; ++kb) {
-    bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
-#pragma unroll
-    for(int i = 0; i < FM; ++i) {
-      const int blk = wm * FM + i;
-      ah[i] = as_bf16x8(sa[(blk * KB_PER_STAGE + kb) * NHL * 64 + lane]);
-      if(NHL == 2) al[i] = as_bf16x8(sa[((blk * KB_PER_STAGE + kb) * NHL + 1) * 64 + lane]);
-    }
-#pragma unroll
-    for(int j = 0; j < FN; ++j) {
-      const int blk = wn * FN + j;
-      bh[j] = as_bf16x8(sb[(blk * KB_PER_STAGE + kb) * NHL * 64 + lane]);
-      if(NHL == 2) bl[j] = as_bf16x8(sb[((blk * KB_PER_STAGE + kb) * NHL + 1) * 64 + lane]);
-    }
-    // Product-major order: consecutive MFMAs write different accumulators,
-    // so none waits on the previous one's result.  The small cross terms go
-    // in first so they are added while the accumulator is smallest.
-    if(NHL == 2) {
-#pragma unroll
-      for(int i = 0; i < FM; ++i)
-#pragma unroll
-        for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-      for(int i = 0; i < FM; ++i)
-#pragma unroll
-        for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-    }
-#pragma unroll
-    for(int i = 0; i < FM; ++i)
-#pragma unroll
-      for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-  }
+  for(int kb = 0; kb < KB_PER_STAGE; ++kb) bf16_kb_mma<FM, FN, NHL>(acc, sa, sb, wm, wn, lane, kb);
 }
 
 template <int FM, int FN>
@@ -374,17 +379,29 @@ __device__ __forceinline__ void bf16_tile_direct(const DenseOp& op, int tm, int 
     bf16_stage_mma<FM, FN, NHL>(acc, sa, sa + SA, wm, wn, lane);
   };
 
-  // As in f32_tile: loads of step s+1 are in flight during the MFMAs of step
-  // s and land in the other buffer afterwards; one barrier per step.
+  // Two-deep pipeline, one barrier per step.  In step s the wave contracts
+  // the first k block of stage s, then splits the registers loaded one step
+  // earlier (stage s+1) into the other LDS buffer and issues the loads of
+  // stage s+2 into the freed registers, then contracts the second k block:
+  // the split runs while the MFMAs of stage s are still in the pipe and the
+  // loads have a whole step to arrive.  Loads past the last stage repeat the
+  // last stage (in bounds, never read), so the loop has no branches.
+  auto compute_kb = [&](int cur, int kb) __attribute__((always_inline)) {
+    const u32x4* sa = lds + cur * (SA + SB);
+    bf16_kb_mma<FM, FN, NHL>(acc, sa, sa + SA, wm, wn, lane, kb);
+  };
   if(steps > 0) {
     issue(0);
     commit(0);
+    issue(steps > 1 ? 1 : 0);
   }
   __syncthreads();
   for(int s = 0; s + 1 < steps; ++s) {
-    issue(s + 1);
-    compute(s & 1);
+    compute_kb(s & 1, 0);
     commit((s + 1) & 1);
+    issue(s + 2 < steps ? s + 2 : steps - 1);
+#pragma unroll
+    for(int kb = 1; kb < KB_PER_STAGE; ++kb) compute_kb(s & 1, kb);
     __syncthreads();
   }
   if(steps > 0) compute((steps - 1) & 1);
