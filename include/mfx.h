@@ -148,7 +148,7 @@ mfx_status mfx_grouped_status(mfx_handle handle, mfx_stream stream);
 
 /* Graph form of mfx_gemm_grouped: captures the grouped launch into a hipGraph
  * once (instantiate) and replays it (launch).  The problem list is copied; the
- * graph owns its work tables and packed panels (device memory freed by
+ * graph owns its work tables and counters (device memory freed by
  * mfx_graph_destroy).  Replays of one graph must not overlap. */
 typedef struct mfx_graph_s* mfx_graph;
 mfx_status mfx_grouped_graph_create(mfx_handle handle, mfx_precision prec,

@@ -12,10 +12,12 @@
 // blk32*32 + l%32 and k = kb64*64 + 32*(l/32) + 16*half + 0..15.  Rows and k
 // are zero-padded to the tile and to 128.  Exponents: E[row/128][k/128].
 //
-// Contraction tiles: 256x256 (waves of 128x128) or 128x128 (waves of 64x64);
+// Contraction tiles: 256x256 (four waves of 128x128, or eight of 128x64) or
+// 128x128 (waves of 64x64);
 // each wave's rows and columns lie inside one 128-block, so its A and B
 // scales are one byte each per stage.  Stage = 128 k (single) or 64 k
-// (two-term), double-buffered in LDS, 16-byte chunks read conflict-free.
+// (two-term), double-buffered in LDS and filled by direct global-to-LDS
+// loads, 16-byte chunks read conflict-free.
 #include "mfx_internal.h"
 
 namespace mfx {
@@ -208,17 +210,22 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_fp8_kernel(co
   const int kb64s = t.kb128s * 2;
   const int steps = T == 1 ? t.kb128s : kb64s;
   const int64_t blk_stride = (int64_t)kb64s * 2 * 64;
-  u32x4 ra[Cfg::PER_A], rb[Cfg::PER_B];
 
-  auto issue = [&](int s) __attribute__((always_inline)) {
+  // Global -> LDS without registers (global_load_lds_dwordx4): for one wave
+  // and one i the 64 lanes' chunks are contiguous both in the packed operand
+  // (RUN and SA / T are multiples of 64) and in the LDS stage.
+  const int wave_base = tid - lane;
+  auto issue = [&](int s, int buf) __attribute__((always_inline)) {
     const int64_t koff = (int64_t)s * RUN;
+    u32x4* sa = lds + buf * (SA + SB);
+    u32x4* sb = sa + SA;
 #pragma unroll
     for(int i = 0; i < Cfg::PER_A; ++i) {
       const int c = tid + i * NT;
       const int term = c / (SA / T), cc = c % (SA / T);
       const int blk = cc / RUN, run = cc - blk * RUN;
       const u32x4* src = term ? t.A2 : t.A1;
-      ra[i] = src[(int64_t)(tm * (BM / 32) + blk) * blk_stride + koff + run];
+      lds_dma16(src + (int64_t)(tm * (BM / 32) + blk) * blk_stride + koff + run, sa + wave_base + i * NT);
     }
 #pragma unroll
     for(int i = 0; i < Cfg::PER_B; ++i) {
@@ -226,28 +233,26 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_fp8_kernel(co
       const int term = c / (SB / T), cc = c % (SB / T);
       const int blk = cc / RUN, run = cc - blk * RUN;
       const u32x4* src = term ? t.B2 : t.B1;
-      rb[i] = src[(int64_t)(tn * (BN / 32) + blk) * blk_stride + koff + run];
+      lds_dma16(src + (int64_t)(tn * (BN / 32) + blk) * blk_stride + koff + run, sb + wave_base + i * NT);
     }
   };
-  auto commit = [&](int buf) __attribute__((always_inline)) {
-    u32x4* sa = lds + buf * (SA + SB);
-    u32x4* sb = sa + SA;
-#pragma unroll
-    for(int i = 0; i < Cfg::PER_A; ++i) sa[tid + i * NT] = ra[i];
-#pragma unroll
-    for(int i = 0; i < Cfg::PER_B; ++i) sb[tid + i * NT] = rb[i];
+  // E8M0 scale bytes (biased exponents) of this wave's A rows and B columns
+  // for stage s, fetched one stage ahead.
+  struct Scales {
+    int a1, b1, a2, b2;
   };
-  auto compute = [&](int s, int cur) __attribute__((always_inline)) {
+  auto scales = [&](int s) __attribute__((always_inline)) {
+    const int kb128 = T == 1 ? s : s / 2;
+    Scales r;
+    r.a1 = t.eA1[arow128 * t.kb128s + kb128] + 127;
+    r.b1 = t.eB1[bcol128 * t.kb128s + kb128] + 127;
+    r.a2 = T == 2 ? t.eA2[arow128 * t.kb128s + kb128] + 127 : 0;
+    r.b2 = T == 2 ? t.eB2[bcol128 * t.kb128s + kb128] + 127 : 0;
+    return r;
+  };
+  auto compute = [&](int cur, const Scales& sc) __attribute__((always_inline)) {
     const u32x4* sa = lds + cur * (SA + SB);
     const u32x4* sb = sa + SA;
-    const int kb128 = T == 1 ? s : s / 2;
-    // E8M0 scale bytes (biased exponents) of this wave's A rows and B columns
-    const int sA1 = t.eA1[arow128 * t.kb128s + kb128] + 127, sB1 = t.eB1[bcol128 * t.kb128s + kb128] + 127;
-    int sA2 = 0, sB2 = 0;
-    if(T == 2) {
-      sA2 = t.eA2[arow128 * t.kb128s + kb128] + 127;
-      sB2 = t.eB2[bcol128 * t.kb128s + kb128] + 127;
-    }
 #pragma unroll
     for(int ks = 0; ks < KS; ++ks) {
       i32x8 a1[FM], a2[FM], b1[FN], b2[FN];
@@ -268,33 +273,39 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_fp8_kernel(co
         for(int i = 0; i < FM; ++i)
 #pragma unroll
           for(int j = 0; j < FN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a2[i], b1[j], acc[i][j], 0, 0, 0, sA2, 0, sB1);
+            acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a2[i], b1[j], acc[i][j], 0, 0, 0, sc.a2, 0, sc.b1);
 #pragma unroll
         for(int i = 0; i < FM; ++i)
 #pragma unroll
           for(int j = 0; j < FN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a1[i], b2[j], acc[i][j], 0, 0, 0, sA1, 0, sB2);
+            acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a1[i], b2[j], acc[i][j], 0, 0, 0, sc.a1, 0, sc.b2);
       }
 #pragma unroll
       for(int i = 0; i < FM; ++i)
 #pragma unroll
         for(int j = 0; j < FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a1[i], b1[j], acc[i][j], 0, 0, 0, sA1, 0, sB1);
+          acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a1[i], b1[j], acc[i][j], 0, 0, 0, sc.a1, 0, sc.b1);
     }
   };
 
+  // Stage s+1 streams into the other buffer (and its scales into registers)
+  // while stage s is contracted; one barrier per stage.
+  Scales cur{127, 127, 127, 127};
   if(steps > 0) {
-    issue(0);
-    commit(0);
+    issue(0, 0);
+    cur = scales(0);
   }
+  lds_dma_wait();
   __syncthreads();
   for(int s = 0; s + 1 < steps; ++s) {
-    issue(s + 1);
-    compute(s, s & 1);
-    commit((s + 1) & 1);
+    issue(s + 1, (s + 1) & 1);
+    const Scales next = scales(s + 1);
+    compute(s & 1, cur);
+    cur = next;
+    lds_dma_wait();
     __syncthreads();
   }
-  if(steps > 0) compute(steps - 1, (steps - 1) & 1);
+  if(steps > 0) compute((steps - 1) & 1, cur);
 
   static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
     constexpr int i = decltype(ij)::value / FN, j = decltype(ij)::value % FN;
@@ -303,10 +314,10 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_fp8_kernel(co
 }
 
 struct Fp8Config {
-  int bm, bn;
+  int bm, bn, waves;  // waves = 8: 128x64 per wave, two waves per SIMD
 };
-constexpr Fp8Config kFp8Configs[] = {{256, 256}, {128, 128}};
-constexpr int kFp8ConfigCount = 2;
+constexpr Fp8Config kFp8Configs[] = {{256, 256, 4}, {128, 128, 4}, {256, 256, 8}};
+constexpr int kFp8ConfigCount = 3;
 
 struct Fp8Plan {
   int64_t rows_pad_a, rows_pad_b, kb128s;
@@ -354,7 +365,7 @@ bool fp8_config_info(int c, int* bm, int* bn, int* bk, int* waves) {
   *bm = kFp8Configs[c].bm;
   *bn = kFp8Configs[c].bn;
   *bk = 128;
-  *waves = 4;
+  *waves = kFp8Configs[c].waves;
   return true;
 }
 
@@ -409,6 +420,11 @@ mfx_status launch_dense_fp8(Handle* h, const DenseOp& op, bool two_term, int con
   t.tiles_n = (int)ceil_div(op.N, c.bn);
   const int grid = t.tiles_m * t.tiles_n;
   const char* name = two_term ? "gemm_fp8_block2_mfma" : "gemm_fp8_block_mfma";
+  if(config == 2) {
+    if(two_term)
+      return timed_launch(h, name, s, [&] { gemm_fp8_kernel<256, 256, 128, 64, 2><<<grid, 512, 0, s>>>(t); });
+    return timed_launch(h, name, s, [&] { gemm_fp8_kernel<256, 256, 128, 64, 1><<<grid, 512, 0, s>>>(t); });
+  }
   if(config == 0) {
     if(two_term)
       return timed_launch(h, name, s, [&] { gemm_fp8_kernel<256, 256, 128, 128, 2><<<grid, 256, 0, s>>>(t); });

@@ -1,5 +1,6 @@
 // The C ABI of include/mfx.h: argument validation, handle and workspace
 // management, kernel timing, and dispatch to the kernel families.
+#include <algorithm>
 #include <cstdio>
 #include <map>
 
@@ -142,25 +143,28 @@ void replace_tuned(const std::map<std::string, int>& t) {
   g_tuned = t;
 }
 
-// Heuristic used when nothing is tuned: the largest tile that still gives
-// every CU at least one tile.
+// Heuristic used when nothing is tuned: the largest tile (more waves first
+// among equal tiles) that still gives every CU at least one tile, else the
+// smallest tile.
 int heuristic_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k) {
   const int cus = h->cu_count > 0 ? h->cu_count : 256;
-  const int count = config_count(p);
-  int best = count > 0 ? count - 1 : 0;
-  // configs 4 (256x256, eight waves), 0 (256x256), 3 (256x128), 1 (128x128),
-  // 2 (64x64) in order of size; precisions without config 4 skip it
-  const int order[5] = {4, 0, 3, 1, 2};
-  for(int oi = 0; oi < 5; ++oi) {
-    const int c = order[oi];
+  struct Cand {
+    int id, bm, bn, waves;
+  };
+  std::vector<Cand> cands;
+  for(int c = 0; c < config_count(p); ++c) {
     int bm, bn, bk, w;
-    if(c >= count || !config_info(p, c, &bm, &bn, &bk, &w)) continue;
-    const int64_t tiles = ceil_div(m, bm) * ceil_div(n, bn);
-    if(tiles >= cus) return c;
-    best = c;
+    if(config_info(p, c, &bm, &bn, &bk, &w)) cands.push_back({c, bm, bn, w});
   }
+  if(cands.empty()) return 0;
+  std::stable_sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) {
+    const int64_t ax = (int64_t)x.bm * x.bn, ay = (int64_t)y.bm * y.bn;
+    return ax != ay ? ax > ay : x.waves > y.waves;
+  });
+  for(const Cand& c : cands)
+    if(ceil_div(m, c.bm) * ceil_div(n, c.bn) >= cus) return c.id;
   (void)k;
-  return best;
+  return cands.back().id;
 }
 
 int choose_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k) {

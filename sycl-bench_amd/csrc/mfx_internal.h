@@ -239,6 +239,19 @@ struct Epilogue {
 };
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// One 16-byte chunk per lane from global memory straight into LDS
+// (global_load_lds_dwordx4): lane l's chunk lands at lds_wave_base + l.  The
+// LDS base must be the same for the whole wave.
+template <class T>
+__device__ __forceinline__ void lds_dma16(const T* src, T* lds_wave_base) {
+  static_assert(sizeof(T) == 16, "one 16-byte chunk per lane");
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+      (__attribute__((address_space(3))) void*)lds_wave_base, 16, 0, 0);
+}
+
+// Waits for this wave's outstanding memory operations, LDS transfers included.
+__device__ __forceinline__ void lds_dma_wait() { __builtin_amdgcn_s_waitcnt(0); }
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // Compile-time loop: f(std::integral_constant<int, I>) for I in [0, N).  Used

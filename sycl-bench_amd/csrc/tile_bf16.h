@@ -122,17 +122,6 @@ struct BfTileArgs {
 
 __device__ __forceinline__ bf16x8 as_bf16x8(u32x4 u) { return __builtin_bit_cast(bf16x8, u); }
 
-// One 16-byte chunk per lane from global memory straight into LDS
-// (global_load_lds_dwordx4): lane l's chunk lands at lds_wave_base + l.  The
-// LDS base must be the same for the whole wave.
-__device__ __forceinline__ void lds_dma16(const u32x4* src, u32x4* lds_wave_base) {
-  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-      (__attribute__((address_space(3))) void*)lds_wave_base, 16, 0, 0);
-}
-
-// Waits for this wave's outstanding memory operations, LDS transfers included.
-__device__ __forceinline__ void lds_dma_wait() { __builtin_amdgcn_s_waitcnt(0); }
-
 // MFMAs of one 16-wide k block `kb` of an LDS stage: acc[i][j] +=
 // A-fragments x B-fragments; sa / sb are the stage's A / B chunks.
 template <int FM, int FN, int NHL>

@@ -4,7 +4,9 @@
 // With seeded inputs it computes the PolyBench 4 form used for the precision
 // study, D = alpha * (A * B) * C + beta * D, as T = A * B; D = alpha*T*C + beta*D.
 // Each contraction is one mfx_gemm call on the device stream, so the second
-// waits for the first by stream order (the reference's accessor DAG edge).
+// waits for the first by stream order (the reference's accessor DAG edge);
+// --grouped submits both as one persistent grouped launch in which the second
+// waits for the first in-kernel.
 #include <string>
 #include <vector>
 
@@ -16,7 +18,7 @@ class Polybench_2mm {
 public:
   explicit Polybench_2mm(const BenchmarkArgs& args)
       : args_(args), size_(args.problem_size), opt_(args), alpha_(args.cli.getOrDefault<float>("--alpha", 1.5f)),
-        beta_(args.cli.getOrDefault<float>("--beta", 1.2f)) {}
+        beta_(args.cli.getOrDefault<float>("--beta", 1.2f)), grouped_(args.cli.isFlagSet("--grouped")) {}
 
   void setup() {
     HipDevice& d = hipDevice(args_);
@@ -49,6 +51,21 @@ public:
             "mfx_gemm");
       });
     };
+    if(grouped_) {
+      mfx_gemm_problem p[2];
+      if(opt_.reference_inputs()) {
+        p[0] = {n, n, n, 1.0f, 1.0f, A_.get(), n, 0, B_.get(), n, 0, C_.get(), n};  // C += A B
+        p[1] = {n, n, n, 1.0f, 0.0f, C_.get(), n, 0, D_.get(), n, 0, E_.get(), n};  // E = C D
+      } else {
+        p[0] = {n, n, n, 1.0f, 0.0f, A_.get(), n, 0, B_.get(), n, 0, E_.get(), n};      // T = A B
+        p[1] = {n, n, n, alpha_, beta_, E_.get(), n, 0, C_.get(), n, 0, D_.get(), n};  // D = a T C + b D
+      }
+      const mfx_dep dep{1, 0};
+      timedSubmit(d, kernels, [&] {
+        mfxCheck(mfx_gemm_grouped(d.handle(), opt_.precision, p, 2, &dep, 1, d.stream()), "mfx_gemm_grouped");
+      });
+      return;
+    }
     if(opt_.reference_inputs()) {
       gemm(1.0f, A_.get(), B_.get(), 1.0f, C_.get());  // Polybench_2mm_1, polybench/2mm.cpp:94-107
       gemm(1.0f, C_.get(), D_.get(), 0.0f, E_.get());  // Polybench_2mm_2, polybench/2mm.cpp:109-123
@@ -85,13 +102,16 @@ public:
     return {4.0 * n * n * n, "FLOP"};
   }
 
-  std::string getBenchmarkName(const BenchmarkArgs&) const { return "Polybench_2mm" + opt_.suffix(); }
+  std::string getBenchmarkName(const BenchmarkArgs&) const {
+    return "Polybench_2mm" + opt_.suffix() + (grouped_ ? "_grouped" : "");
+  }
 
 private:
   BenchmarkArgs args_;
   std::size_t size_;
   ContractionOptions opt_;
   float alpha_, beta_;
+  bool grouped_;
   DeviceBuffer<float> A_, B_, C_, D_, E_, C0_, D0_;
   CheckResult check_;
 };

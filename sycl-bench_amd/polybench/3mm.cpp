@@ -1,6 +1,7 @@
 // Polybench_3mm: E = A * B, F = C * D, G = E * F (polybench/3mm.cpp:112-159).
 // Three mfx_gemm calls in stream order; the first two are independent, the
-// third reads both (the reference's diamond DAG).
+// third reads both (the reference's diamond DAG).  --grouped submits the three
+// as one persistent grouped launch with the DAG as in-kernel dependencies.
 #include <string>
 #include <vector>
 
@@ -10,7 +11,8 @@
 
 class Polybench_3mm {
 public:
-  explicit Polybench_3mm(const BenchmarkArgs& args) : args_(args), size_(args.problem_size), opt_(args) {}
+  explicit Polybench_3mm(const BenchmarkArgs& args)
+      : args_(args), size_(args.problem_size), opt_(args), grouped_(args.cli.isFlagSet("--grouped")) {}
 
   void setup() {
     HipDevice& d = hipDevice(args_);
@@ -43,6 +45,17 @@ public:
             "mfx_gemm");
       });
     };
+    if(grouped_) {
+      const mfx_gemm_problem p[3] = {
+          {n, n, n, 1.0f, 0.0f, A_.get(), n, 0, B_.get(), n, 0, E_.get(), n},   // Polybench_3mm_1
+          {n, n, n, 1.0f, 0.0f, C_.get(), n, 0, D_.get(), n, 0, F_.get(), n},   // Polybench_3mm_2
+          {n, n, n, 1.0f, 0.0f, E_.get(), n, 0, F_.get(), n, 0, G_.get(), n}};  // Polybench_3mm_3
+      const mfx_dep deps[2] = {{2, 0}, {2, 1}};
+      timedSubmit(d, kernels, [&] {
+        mfxCheck(mfx_gemm_grouped(d.handle(), opt_.precision, p, 3, deps, 2, d.stream()), "mfx_gemm_grouped");
+      });
+      return;
+    }
     gemm(A_.get(), B_.get(), E_.get());  // Polybench_3mm_1
     gemm(C_.get(), D_.get(), F_.get());  // Polybench_3mm_2
     gemm(E_.get(), F_.get(), G_.get());  // Polybench_3mm_3
@@ -70,12 +83,15 @@ public:
     return {6.0 * n * n * n, "FLOP"};
   }
 
-  std::string getBenchmarkName(const BenchmarkArgs&) const { return "Polybench_3mm" + opt_.suffix(); }
+  std::string getBenchmarkName(const BenchmarkArgs&) const {
+    return "Polybench_3mm" + opt_.suffix() + (grouped_ ? "_grouped" : "");
+  }
 
 private:
   BenchmarkArgs args_;
   std::size_t size_;
   ContractionOptions opt_;
+  bool grouped_;
   DeviceBuffer<float> A_, B_, C_, D_, E_, F_, G_;
   CheckResult check_;
 };

@@ -92,6 +92,16 @@ def test_matmulchain_grouped_launch(init):
     assert res["_names"][0].endswith("_len8_grouped")
 
 
+@pytest.mark.parametrize("exe", ["2mm", "3mm"])
+@pytest.mark.parametrize("init,prec", [("polybench", "fp32"), ("uniform", "split-bf16"), ("uniform", "fp32")])
+def test_2mm_3mm_grouped_launch(exe, init, prec):
+    r, res = bench(exe, "--size=384", "--num-runs=2", f"--init={init}", f"--precision={prec}", "--grouped",
+                   "--metrics=extended")
+    assert r.returncode == 0, r.stderr
+    assert res["Verification"] == "PASS", res
+    assert res["_names"][0].endswith("_grouped")
+
+
 def test_csv_output_header(tmp_path):
     csv = tmp_path / "out.csv"
     r, _ = bench("gemm", "--size=256", "--num-runs=2", f"--output={csv}")

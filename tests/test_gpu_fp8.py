@@ -46,10 +46,7 @@ def two_term(oracle, x):
     return dequant(oracle, q1, e1), dequant(oracle, q2, e2)
 
 
-@pytest.mark.parametrize("prec", ["fp8-block", "fp8-block2"])
-@pytest.mark.parametrize("m,n,k,tb", [(256, 256, 256, False), (384, 200, 300, False), (128, 520, 640, True),
-                                      (1024, 1024, 1024, False)])
-def test_fp8_contraction_equals_dequantised_product(handle, oracle, prec, m, n, k, tb):
+def check_dequantised_product(handle, oracle, prec, m, n, k, tb):
     rng = np.random.default_rng(m * 7 + n + k)
     A = lognormal(rng, (m, k), 1.0)
     B = lognormal(rng, (n, k) if tb else (k, n), 1.0)
@@ -71,6 +68,23 @@ def test_fp8_contraction_equals_dequantised_product(handle, oracle, prec, m, n, 
     err = np.linalg.norm(got - want) / np.linalg.norm(want)
     # the scaled FP8 MFMA sums each 64-wide dot with ~2^-15 relative error (measured)
     assert err < 1e-4, err
+
+
+@pytest.mark.parametrize("prec", ["fp8-block", "fp8-block2"])
+@pytest.mark.parametrize("m,n,k,tb", [(256, 256, 256, False), (384, 200, 300, False), (128, 520, 640, True),
+                                      (1024, 1024, 1024, False)])
+def test_fp8_contraction_equals_dequantised_product(handle, oracle, prec, m, n, k, tb):
+    check_dequantised_product(handle, oracle, prec, m, n, k, tb)
+
+
+@pytest.mark.parametrize("prec", ["fp8-block", "fp8-block2"])
+def test_fp8_every_tile_configuration(handle, oracle, mfx, prec):
+    try:
+        for config in range(mfx.config_count(prec)):
+            handle.force_config(prec, config)
+            check_dequantised_product(handle, oracle, prec, 520, 300, 384, config % 2 == 1)
+    finally:
+        handle.force_config(prec, -1)
 
 
 def test_fp8_error_vs_fp64_is_reported_and_two_term_helps(handle):
