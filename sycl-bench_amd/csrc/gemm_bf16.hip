@@ -85,8 +85,10 @@ PackPlan plan(const DenseOp& op, bool split, int config) {
   const BfConfig& c = kBfConfigs[config];
   PackPlan pl;
   const int nhl = split ? 2 : 1;
-  pl.rows_pad_a = round_up(op.M, c.bm);
-  pl.rows_pad_b = round_up(op.N, c.bn);
+  // rows padded to 32 only (bf16_tile reads the last block again past them)
+  pl.rows_pad_a = round_up(op.M, 32);
+  pl.rows_pad_b = round_up(op.N, 32);
+  (void)c;
   pl.kbs = round_up(op.K, 16 * KB_PER_STAGE) / 16;
   pl.bytes_a = (size_t)pl.rows_pad_a * pl.kbs * 16 * nhl * 2;
   pl.bytes_b = (size_t)pl.rows_pad_b * pl.kbs * 16 * nhl * 2;
@@ -165,8 +167,10 @@ mfx_status launch_dense_bf16x(Handle* h, const DenseOp& op, bool split, int conf
   p.t.kbs = (int)pl.kbs;
   p.t.nprod = op.nprod;
   p.t.ep = op.ep;
-  p.tiles_m = (int)(pl.rows_pad_a / c.bm);
-  p.tiles_n = (int)(pl.rows_pad_b / c.bn);
+  p.t.nblk_a = (int)(pl.rows_pad_a / 32);
+  p.t.nblk_b = (int)(pl.rows_pad_b / 32);
+  p.tiles_m = (int)ceil_div(op.M, c.bm);
+  p.tiles_n = (int)ceil_div(op.N, c.bn);
   p.group = 8;
   p.mode = op.tiles;
   int grid;

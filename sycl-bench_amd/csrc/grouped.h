@@ -1,13 +1,21 @@
 // Persistent grouped launch: tables shared by the host planner (grouped.hip)
 // and the kernels (grouped_bf16.hip, grouped_f32.hip).
 //
-// The host turns the problem list into a work queue of items, one per output
-// tile, in topological order: level by level (a problem's level is 1 + the
-// highest level of its producers), largest K first.  A fixed grid of one
-// workgroup per CU pulls items from an atomic head counter.  Before working
-// on a tile it waits, with a bounded spin, for every producer of its problem
-// to have finished all its tiles.  Tiles read the FP32 operands directly; the
-// BF16 modes split them while staging (bf16_tile_direct).
+// The host turns the problem list into a work queue of items in topological
+// order, level by level (a problem's level is 1 + the highest level of its
+// producers):
+//   PACK_A / PACK_B (split / single BF16 only): split one 32-row block x
+//       PACK_KBLOCKS k blocks of an operand into the fragment order of
+//       tile_bf16.h, in the workspace;
+//   TILE: one output tile of one problem.
+// Each level lists its pack items, then its tiles, largest K first.  A fixed
+// grid of one workgroup per CU pulls items from an atomic head counter.
+// Before working on an item it waits, with a bounded spin, for
+//   - PACK: every producer of the problem to have finished all its tiles;
+//   - TILE (BF16): every pack item of its A and B panels (a counter per
+//     panel); TILE (FP32, which reads the operands directly): the producers.
+// Pack items are small (about 1-2 us each) so a level's packing spreads over
+// all workgroups at HBM speed instead of one workgroup per panel.
 // Completion is published by fence + barrier + atomic increment; waits are
 // acquire loads at agent scope, so producer results written on another XCD
 // are visible.  Deadlock freedom: a workgroup only waits for items that
@@ -30,6 +38,10 @@ namespace mfx {
 
 constexpr int GROUPED_MAX_DEPS = 4;
 
+constexpr int PACK_KBLOCKS = 32;  // 16-wide k blocks per pack item (512 k)
+
+enum GroupedItemType : int { ITEM_PACK_A = 0, ITEM_PACK_B = 1, ITEM_TILE = 2 };
+
 struct GProblem {
   int M, N, K;
   float alpha, beta;
@@ -43,12 +55,30 @@ struct GProblem {
   int vec_a, vec_b;
   int config;  // tile configuration of this problem
   int tiles_m, tiles_n;
+  int kbs;                    // BF16: padded K / 16
+  int pack_slices;            // BF16: pack items per 32-row block (ceil(kbs / PACK_KBLOCKS))
+  u32x4* packed_a;            // BF16: fragment-order operands
+  u32x4* packed_b;
+  int pack_a_ctr, pack_b_ctr; // BF16: first panel counter of A / B
+  int prepacked;              // BF16: operands packed by the pre-pass kernel
+  int consumed;               // another problem waits for this one's tiles
   int ndeps;
   int deps[GROUPED_MAX_DEPS];  // producer problem indices
 };
 
+// One operand packed by the pre-pass kernel (problems without producers):
+// chunks [first_block*256, (first_block+blocks)*256) of the flattened job list.
+struct PackJob {
+  PackSrc src;
+  u32x4* dst;
+  int kbs;
+  int blocks;       // 256-chunk blocks of this job (the last one partly used)
+  int64_t chunks;   // rows_pad / 32 * kbs * 64
+  int first_block;
+};
+
 struct GItem {
-  int problem, x, y, pad;  // output tile (x, y) = (tm, tn) of `problem`
+  int type, problem, x, y;  // TILE: (x, y) = (tm, tn); PACK: x = 32-row block, y = k slice
 };
 
 struct GroupedArgs {
@@ -58,6 +88,7 @@ struct GroupedArgs {
   int* head;        // next item to take
   int* error;       // set to 1 when a wait times out
   int* done;        // per problem: finished tiles
+  int* pack_ctr;    // per packed panel: finished pack items
   int spin_limit;
 };
 
@@ -70,19 +101,26 @@ template <> struct Geo<1> { static constexpr int BM = 128, BN = 128, WM = 64, WN
 template <> struct Geo<2> { static constexpr int BM = 64, BN = 64, WM = 32, WN = 32, BK = 32; };
 template <> struct Geo<3> { static constexpr int BM = 256, BN = 128, WM = 128, WN = 64, BK = 16; };
 
-inline int host_bm(int config) { return config == 1 ? 128 : config == 2 ? 64 : 256; }
-inline int host_bn(int config) { return config == 0 ? 256 : config == 2 ? 64 : 128; }
+__host__ __device__ inline int cfg_bm(int config) { return config == 1 ? 128 : config == 2 ? 64 : 256; }
+__host__ __device__ inline int cfg_bn(int config) { return config == 0 ? 256 : config == 2 ? 64 : 128; }
+
+// Cross-XCD ordering costs cache maintenance on gfx950: an agent-scope
+// release writes back the XCD's L2 (buffer_wbl2) and an agent-scope acquire
+// invalidates it (buffer_inv).  So waits poll with relaxed atomic loads
+// (performed at the coherent level, no invalidation) and acquire once when
+// the condition holds, and a signal writes back once per workgroup.
 
 // Bounded wait for *ctr >= target (all threads of the workgroup).
 __device__ __forceinline__ bool wait_at_least(int* ctr, int target, const GroupedArgs& g) {
   int spins = 0;
-  while(__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+  while(__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
     if(++spins > g.spin_limit) {
       __hip_atomic_store(g.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
     }
     __builtin_amdgcn_s_sleep(8);
   }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return true;
 }
 
@@ -95,11 +133,17 @@ __device__ __forceinline__ bool wait_producers(const GProblem& P, const GroupedA
   return ok;
 }
 
-// Publishes the workgroup's global writes, then bumps *ctr once.
+// Publishes the workgroup's global writes, then bumps *ctr once: every thread
+// waits for its own stores to reach L2, the barrier orders them before
+// thread 0, whose agent-scope release writes the L2 back before the add.
 __device__ __forceinline__ void signal(int* ctr) {
-  __threadfence();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if(threadIdx.x == 0) __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if(threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // Next item index for the whole workgroup.  The value is the same in every
@@ -111,6 +155,9 @@ __device__ __forceinline__ int take_item(int* head, int* slot) {
   __syncthreads();
   return __builtin_amdgcn_readfirstlane(*slot);
 }
+
+// Pre-pass: packs every job's operand (one chunk per thread).
+mfx_status launch_pack_jobs(Handle* h, int nhl, const PackJob* jobs, int njobs, int blocks, hipStream_t s);
 
 // Kernel launchers (one instantiation per pair of configurations).
 // vec: every problem allows 128-bit operand loads.

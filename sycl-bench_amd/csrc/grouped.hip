@@ -17,19 +17,23 @@ namespace {
 
 // Tile policies: (large, small, bound) -- a problem takes the large tile when
 // that pads its output area by at most `bound` (0: always), else the small
-// one.  The grouped kernels do not compile the 256x256 configuration (0).
+// one.  The FP32 grouped kernels do not compile the 256x256 configuration (0)
+// and run 256x128 (3) in its place.
 struct GroupedPolicy {
   int big, small;
   double bound;
 };
 constexpr GroupedPolicy kPolicies[] = {
-    {3, 3, 0.0},   // 0
-    {3, 1, 1.34},  // 1
-    {3, 1, 2.0},   // 2
-    {3, 2, 1.34},  // 3
-    {1, 1, 0.0},   // 4
-    {1, 2, 1.34},  // 5
-    {2, 2, 0.0},   // 6
+    {0, 0, 0.0},   // 0
+    {0, 1, 1.34},  // 1
+    {0, 2, 1.34},  // 2
+    {3, 3, 0.0},   // 3
+    {3, 1, 1.34},  // 4
+    {3, 1, 2.0},   // 5
+    {3, 2, 1.34},  // 6
+    {1, 1, 0.0},   // 7
+    {1, 2, 1.34},  // 8
+    {2, 2, 0.0},   // 9
 };
 constexpr int kPolicyCount = sizeof(kPolicies) / sizeof(kPolicies[0]);
 
@@ -38,10 +42,13 @@ constexpr int kPolicyCount = sizeof(kPolicies) / sizeof(kPolicies[0]);
 // untuned policy estimate.
 constexpr double kTileRate[4] = {1.0, 0.6, 0.2, 0.85};
 
-int problem_config(const GroupedPolicy& pol, int64_t m, int64_t n) {
-  const int big = pol.big, small = pol.small;
+// The configuration a grouped launch of `prec` runs for configuration c.
+int grouped_config(mfx_precision prec, int c) { return prec == MFX_PREC_FP32 && c == 0 ? 3 : c; }
+
+int problem_config(const GroupedPolicy& pol, mfx_precision prec, int64_t m, int64_t n) {
+  const int big = grouped_config(prec, pol.big), small = grouped_config(prec, pol.small);
   if(big == small || pol.bound <= 0.0) return big;
-  const double padded = (double)round_up(m, host_bm(big)) * (double)round_up(n, host_bn(big));
+  const double padded = (double)round_up(m, cfg_bm(big)) * (double)round_up(n, cfg_bn(big));
   return padded <= pol.bound * (double)m * (double)n ? big : small;
 }
 
@@ -57,7 +64,7 @@ std::vector<int> problem_levels(int count, const mfx_dep* deps, int ndeps) {
 
 // Estimated duration (arbitrary units) of a launch under a policy: per level,
 // the padded tile work spread over the CUs plus half a tile of tail.
-double policy_estimate(const GroupedPolicy& pol, const mfx_gemm_problem* problems, int count,
+double policy_estimate(const GroupedPolicy& pol, mfx_precision prec, const mfx_gemm_problem* problems, int count,
     const std::vector<int>& level, int cus) {
   const int levels = count ? *std::max_element(level.begin(), level.end()) + 1 : 0;
   std::vector<double> work(levels, 0.0), tmax(levels, 0.0);
@@ -65,9 +72,9 @@ double policy_estimate(const GroupedPolicy& pol, const mfx_gemm_problem* problem
   for(int i = 0; i < count; ++i) {
     const auto& q = problems[i];
     if(q.m <= 0 || q.n <= 0) continue;
-    const int c = problem_config(pol, q.m, q.n);
-    const int64_t tiles = ceil_div(q.m, host_bm(c)) * ceil_div(q.n, host_bn(c));
-    const double t = (double)host_bm(c) * host_bn(c) * (double)round_up(std::max<int64_t>(q.k, 1), 32) / kTileRate[c];
+    const int c = problem_config(pol, prec, q.m, q.n);
+    const int64_t tiles = ceil_div(q.m, cfg_bm(c)) * ceil_div(q.n, cfg_bn(c));
+    const double t = (double)cfg_bm(c) * cfg_bn(c) * (double)round_up(std::max<int64_t>(q.k, 1), 32) / kTileRate[c];
     work[level[i]] += t * tiles;
     tmax[level[i]] = std::max(tmax[level[i]], t);
     ntiles[level[i]] += tiles;
@@ -110,7 +117,7 @@ int choose_grouped_policy(Handle* h, mfx_precision prec, const mfx_gemm_problem*
   if(h->grouped_policy >= 0) return h->grouped_policy;
   int forced = h->forced_config[prec];
   if(forced >= 0) {
-    if(forced == 0 || forced == 4) forced = 3;  // no grouped 256x256 kernel
+    if(forced == 4) forced = 0;  // the grouped kernels run four waves
     for(int i = 0; i < kPolicyCount; ++i)
       if(kPolicies[i].big == forced && kPolicies[i].small == forced) return i;
   }
@@ -121,7 +128,7 @@ int choose_grouped_policy(Handle* h, mfx_precision prec, const mfx_gemm_problem*
   int best = 0;
   double best_est = -1.0;
   for(int i = 0; i < kPolicyCount; ++i) {
-    const double e = policy_estimate(kPolicies[i], problems, count, level, cus);
+    const double e = policy_estimate(kPolicies[i], prec, problems, count, level, cus);
     if(best_est < 0 || e < best_est) {
       best_est = e;
       best = i;
@@ -139,6 +146,13 @@ __global__ void zero_ints_kernel(int* p, int n, int* err) {
   if(blockIdx.x == 0 && threadIdx.x == 0) *err = 0;
 }
 
+// Bytes of the packed A and B copies of one problem (each rounded to 256).
+std::pair<size_t, size_t> packed_sizes(const GProblem& P, int nhl) {
+  const size_t ba = (size_t)round_up(P.M, 32) * P.kbs * 16 * nhl * 2;
+  const size_t bb = (size_t)round_up(P.N, 32) * P.kbs * 16 * nhl * 2;
+  return {(size_t)round_up(ba, 256), (size_t)round_up(bb, 256)};
+}
+
 }  // namespace
 
 // A grouped launch prepared on the host: tables, workspace layout and the
@@ -148,7 +162,11 @@ struct GroupedPlan {
   std::vector<GProblem> problems;
   std::vector<GItem> items;
   size_t counter_ints = 0;
-  size_t off_problems = 0, off_items = 0, off_counters = 0;
+  size_t off_problems = 0, off_items = 0, off_jobs = 0, off_counters = 0, off_packed = 0;
+  std::vector<std::pair<int, int>> job_list;  // (problem, 1 = A / 0 = B) packed by the pre-pass
+  std::vector<PackJob> jobs;                  // filled in bind_grouped
+  int pack_blocks = 0;                        // grid of the pre-pass
+  const PackJob* dev_jobs = nullptr;          // the job table in device memory
   size_t total_bytes = 0;
   int policy = 0, big = 0, small = 0;  // tile policy and its configurations
   bool all_vec = true;                 // every problem allows 128-bit operand loads
@@ -193,7 +211,7 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
   pl.prec = prec;
   pl.problems.resize(count);
   std::vector<int> level(count, 0);
-  for(int i = 0; i < count; ++i) pl.problems[i].ndeps = 0;
+  for(int i = 0; i < count; ++i) pl.problems[i].ndeps = pl.problems[i].consumed = 0;
   for(int d = 0; d < ndeps; ++d) {
     GProblem& c = pl.problems[deps[d].consumer];
     bool dup = false;
@@ -201,13 +219,18 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
     if(dup) continue;
     if(c.ndeps == GROUPED_MAX_DEPS) return unsupported("a problem may depend on at most 4 producers");
     c.deps[c.ndeps++] = deps[d].producer;
+    pl.problems[deps[d].producer].consumed = 1;
   }
   for(int i = 0; i < count; ++i)
     for(int e = 0; e < pl.problems[i].ndeps; ++e) level[i] = std::max(level[i], level[pl.problems[i].deps[e]] + 1);
   pl.policy = choose_grouped_policy(h, prec, problems, count, deps, ndeps);
   const GroupedPolicy& pol = kPolicies[pl.policy];
-  pl.big = pol.big;
-  pl.small = pol.small;
+  pl.big = grouped_config(prec, pol.big);
+  pl.small = grouped_config(prec, pol.small);
+  const bool packed = prec != MFX_PREC_FP32;
+  const int nhl = prec == MFX_PREC_SPLIT_BF16 ? 2 : 1;
+  size_t packed_bytes = 0;
+  int panels = 0;
 
   for(int i = 0; i < count; ++i) {
     const auto& q = problems[i];
@@ -225,41 +248,110 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
     P.ldc = q.ldc;
     P.a_kmajor = q.transA == 0;
     P.b_kmajor = q.transB == 1;
-    P.config = problem_config(pol, q.m, q.n);
-    const int bm = host_bm(P.config), bn = host_bn(P.config);
+    P.config = problem_config(pol, prec, q.m, q.n);
+    const int bm = cfg_bm(P.config), bn = cfg_bn(P.config);
     P.tiles_m = (int)ceil_div(q.m, bm);
     P.tiles_n = (int)ceil_div(q.n, bn);
     if(q.m == 0 || q.n == 0) P.tiles_m = P.tiles_n = 0;
     P.vec_a = f32_vec_ok(q.A, q.lda, P.a_kmajor, P.M, P.K);
     P.vec_b = f32_vec_ok(q.B, q.ldb, P.b_kmajor, P.N, P.K);
     if(P.tiles_m * P.tiles_n > 0) pl.all_vec = pl.all_vec && P.vec_a && P.vec_b;
+    P.kbs = 0;
+    P.pack_slices = 0;
+    P.packed_a = P.packed_b = nullptr;
+    P.pack_a_ctr = P.pack_b_ctr = 0;
+    P.prepacked = 0;
+    if(packed && P.tiles_m * P.tiles_n > 0) {
+      // load_chunk8 takes 128-bit loads along k only
+      P.vec_a = P.a_kmajor && P.vec_a;
+      P.vec_b = P.b_kmajor && P.vec_b;
+      P.kbs = (int)(round_up(q.k, 16 * KB_PER_STAGE) / 16);
+      P.pack_slices = (int)ceil_div(P.kbs, PACK_KBLOCKS);
+      packed_bytes += packed_sizes(P, nhl).first + packed_sizes(P, nhl).second;
+      if(P.ndeps == 0) {
+        // inputs only: packed by the pre-pass kernel before the tiles start
+        P.prepacked = 1;
+        if(P.kbs > 0) {
+          pl.job_list.push_back({i, 1});
+          pl.job_list.push_back({i, 0});
+        }
+      } else {
+        P.pack_a_ctr = panels;
+        panels += P.tiles_m;
+        P.pack_b_ctr = panels;
+        panels += P.tiles_n;
+      }
+    }
     pl.flops += 2.0 * q.m * q.n * q.k;
   }
 
-  // Items level by level, largest K first, a problem's tiles row by row.
+  // Items level by level: the level's pack items (largest K first, so the
+  // first tiles' panels are ready first), then its tiles, largest K first, a
+  // problem's tiles row by row.
   const int max_level = count ? *std::max_element(level.begin(), level.end()) : 0;
   for(int lv = 0; lv <= max_level; ++lv) {
     std::vector<int> at;
     for(int i = 0; i < count; ++i)
       if(level[i] == lv && pl.problems[i].tiles_m * pl.problems[i].tiles_n > 0) at.push_back(i);
     std::stable_sort(at.begin(), at.end(), [&](int a, int b) { return pl.problems[a].K > pl.problems[b].K; });
+    if(packed)
+      for(int i : at) {
+        const GProblem& P = pl.problems[i];
+        if(P.prepacked) continue;
+        for(int b = 0; b < (int)ceil_div(P.M, 32); ++b)
+          for(int sl = 0; sl < P.pack_slices; ++sl) pl.items.push_back({ITEM_PACK_A, i, b, sl});
+        for(int b = 0; b < (int)ceil_div(P.N, 32); ++b)
+          for(int sl = 0; sl < P.pack_slices; ++sl) pl.items.push_back({ITEM_PACK_B, i, b, sl});
+      }
     for(int i : at)
       for(int tm = 0; tm < pl.problems[i].tiles_m; ++tm)
-        for(int tn = 0; tn < pl.problems[i].tiles_n; ++tn) pl.items.push_back({i, tm, tn, 0});
+        for(int tn = 0; tn < pl.problems[i].tiles_n; ++tn) pl.items.push_back({ITEM_TILE, i, tm, tn});
   }
 
   pl.off_problems = 0;
   pl.off_items = round_up(sizeof(GProblem) * std::max(1, count), 256);
-  pl.off_counters = pl.off_items + round_up(sizeof(GItem) * std::max<size_t>(1, pl.items.size()), 256);
-  pl.counter_ints = 2 + (size_t)count;
-  pl.total_bytes = pl.off_counters + round_up(pl.counter_ints * sizeof(int), 256);
+  pl.off_jobs = pl.off_items + round_up(sizeof(GItem) * std::max<size_t>(1, pl.items.size()), 256);
+  pl.off_counters = pl.off_jobs + round_up(sizeof(PackJob) * std::max<size_t>(1, pl.job_list.size()), 256);
+  pl.counter_ints = 2 + (size_t)count + (size_t)panels;
+  pl.off_packed = pl.off_counters + round_up(pl.counter_ints * sizeof(int), 256);
+  pl.total_bytes = pl.off_packed + packed_bytes;
   return MFX_SUCCESS;
 }
 
-// Uploads the tables into the device buffer `ws` (stream-ordered, from
-// pinned staging memory owned by the handle) and fills the kernel arguments.
+// Places the packed operands in the device buffer `ws`, uploads the tables
+// (stream-ordered, from pinned staging memory owned by the handle) and fills
+// the kernel arguments.
 static mfx_status bind_grouped(Handle* h, GroupedPlan& pl, void* ws, GroupedArgs& g, hipStream_t s) {
   char* base = static_cast<char*>(ws);
+  if(pl.prec != MFX_PREC_FP32) {
+    const int nhl = pl.prec == MFX_PREC_SPLIT_BF16 ? 2 : 1;
+    size_t off = pl.off_packed;
+    for(auto& P : pl.problems) {
+      if(P.kbs == 0) continue;
+      const auto sz = packed_sizes(P, nhl);
+      P.packed_a = reinterpret_cast<u32x4*>(base + off);
+      off += sz.first;
+      P.packed_b = reinterpret_cast<u32x4*>(base + off);
+      off += sz.second;
+    }
+    pl.jobs.clear();
+    int blocks = 0;
+    for(const auto& jb : pl.job_list) {
+      const GProblem& P = pl.problems[jb.first];
+      const bool is_a = jb.second == 1;
+      PackJob j;
+      j.src = make_pack_src(is_a ? P.A : P.B, is_a ? P.lda : P.ldb, is_a ? P.M : P.N, P.K,
+          is_a ? P.a_kmajor : P.b_kmajor);
+      j.dst = is_a ? P.packed_a : P.packed_b;
+      j.kbs = P.kbs;
+      j.chunks = ceil_div(is_a ? P.M : P.N, 32) * P.kbs * 64;
+      j.blocks = (int)ceil_div(j.chunks, 256);
+      j.first_block = blocks;
+      blocks += j.blocks;
+      pl.jobs.push_back(j);
+    }
+    pl.pack_blocks = blocks;
+  }
   // The previous upload from the staging buffer must have completed.
   if(h->staging_ready) MFX_HIP(hipEventSynchronize(h->staging_ready));
   if(h->staging_bytes < pl.off_counters) {
@@ -273,15 +365,18 @@ static mfx_status bind_grouped(Handle* h, GroupedPlan& pl, void* ws, GroupedArgs
   char* st = static_cast<char*>(h->staging);
   memcpy(st + pl.off_problems, pl.problems.data(), sizeof(GProblem) * pl.problems.size());
   memcpy(st + pl.off_items, pl.items.data(), sizeof(GItem) * pl.items.size());
+  if(!pl.jobs.empty()) memcpy(st + pl.off_jobs, pl.jobs.data(), sizeof(PackJob) * pl.jobs.size());
   MFX_HIP(hipMemcpyAsync(base, st, pl.off_counters, hipMemcpyHostToDevice, s));
   MFX_HIP(hipEventRecord(h->staging_ready, s));
   int* counters = reinterpret_cast<int*>(base + pl.off_counters);
   g.problems = reinterpret_cast<const GProblem*>(base + pl.off_problems);
   g.items = reinterpret_cast<const GItem*>(base + pl.off_items);
+  pl.dev_jobs = reinterpret_cast<const PackJob*>(base + pl.off_jobs);
   g.nitems = (int)pl.items.size();
   g.head = counters;
   g.error = counters + 1;
   g.done = counters + 2;
+  g.pack_ctr = counters + 2 + pl.problems.size();
   g.spin_limit = h->spin_limit;
   return MFX_SUCCESS;
 }
@@ -293,6 +388,9 @@ static mfx_status launch_grouped_kernels(Handle* h, const GroupedPlan& pl, const
     zero_ints_kernel<<<(int)std::min<int64_t>(ceil_div(n, 256), 1024), 256, 0, s>>>(g.head, n, g.error);
   }));
   if(pl.items.empty()) return MFX_SUCCESS;
+  if(pl.prec != MFX_PREC_FP32)
+    MFX_CHECK(launch_pack_jobs(h, pl.prec == MFX_PREC_SPLIT_BF16 ? 2 : 1, pl.dev_jobs, (int)pl.jobs.size(),
+        pl.pack_blocks, s));
   const int grid = h->cu_count > 0 ? h->cu_count : 256;
   if(pl.prec == MFX_PREC_FP32) return launch_grouped_f32(h, pl.big, pl.small, pl.all_vec, g, grid, s);
   return launch_grouped_bf16(h, pl.prec == MFX_PREC_SPLIT_BF16 ? 2 : 1, pl.big, pl.small, pl.all_vec, g, grid, s);
@@ -351,7 +449,7 @@ using namespace mfx;
 
 struct mfx_graph_s {
   int device = 0;
-  void* mem = nullptr;  // tables and counters of this graph
+  void* mem = nullptr;  // tables, counters and packed operands of this graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int* error = nullptr;
@@ -407,7 +505,7 @@ mfx_status mfx_grouped_graph_create(mfx_handle handle, mfx_precision prec, const
   auto* gr = new(std::nothrow) mfx_graph_s;
   if(!gr) return MFX_ERROR_ALLOC;
   gr->device = h->device;
-  // The graph owns its tables and counters, so later calls on
+  // The graph owns its tables, counters and packed operands, so later calls on
   // the handle (which may regrow the handle's workspace) cannot move them.
   hipError_t ae = hipMalloc(&gr->mem, std::max<size_t>(pl.total_bytes, 256));
   if(ae != hipSuccess) {

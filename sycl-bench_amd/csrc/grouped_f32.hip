@@ -52,7 +52,7 @@ __global__ __launch_bounds__(256) void grouped_f32_kernel(const GroupedArgs g) {
       if(P.config == BIG) run_tile<BIG, VEC>(op, P, item.x, item.y, smem);
       else run_tile<SMALL, VEC>(op, P, item.x, item.y, smem);
     }
-    signal(&g.done[item.problem]);
+    if(P.consumed) signal(&g.done[item.problem]);
   }
 }
 

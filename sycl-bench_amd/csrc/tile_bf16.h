@@ -71,13 +71,11 @@ struct PackSrc {
 // Host-side constructor (decides whether 128-bit loads are allowed).
 PackSrc make_pack_src(const float* src, int64_t ld, int rows, int K, bool kmajor);
 
-// Packs lane-chunk (blk, kb, lane): 8 consecutive k of one row, hi (and lo).
-// `dst` points at the packed operand; kbs = number of 16-wide k blocks.
-template <int NHL>
-__device__ __forceinline__ void pack_chunk(const PackSrc& s, int blk, int kb, int lane, int kbs, u32x4* dst) {
+// Loads lane-chunk (blk, kb, lane) of an operand: 8 consecutive k of row
+// blk*32 + lane%32 starting at k = kb*16 + 8*(lane/32); zeros outside it.
+__device__ __forceinline__ void load_chunk8(const PackSrc& s, int blk, int kb, int lane, float (&v)[8]) {
   const int r = blk * 32 + (lane & 31);
   const int k0 = kb * 16 + (lane >> 5) * 8;
-  float v[8];
   if(s.kmajor) {
     const float* p = s.src + (int64_t)r * s.ld + k0;
     if(r < s.rows && s.vec && k0 + 8 <= s.K) {
@@ -93,11 +91,25 @@ __device__ __forceinline__ void pack_chunk(const PackSrc& s, int blk, int kb, in
 #pragma unroll
     for(int e = 0; e < 8; ++e) v[e] = (r < s.rows && k0 + e < s.K) ? s.src[(int64_t)(k0 + e) * s.ld + r] : 0.0f;
   }
+}
+
+// Splits a loaded lane-chunk and stores it at its fragment-order position in
+// the packed operand `dst` (kbs = number of 16-wide k blocks).
+template <int NHL>
+__device__ __forceinline__ void store_chunk(u32x4* dst, int blk, int kb, int lane, int kbs, const float (&v)[8]) {
   const int64_t base = ((int64_t)blk * kbs + kb) * NHL * 64 + lane;
   u32x4 h, l;
   split_chunk<NHL>(v, h, l);
   dst[base] = h;
   if(NHL == 2) dst[base + 64] = l;
+}
+
+// Packs lane-chunk (blk, kb, lane): 8 consecutive k of one row, hi (and lo).
+template <int NHL>
+__device__ __forceinline__ void pack_chunk(const PackSrc& s, int blk, int kb, int lane, int kbs, u32x4* dst) {
+  float v[8];
+  load_chunk8(s, blk, kb, lane, v);
+  store_chunk<NHL>(dst, blk, kb, lane, kbs, v);
 }
 
 template <int BM, int BN, int WM, int WN, int NHL>
@@ -116,6 +128,8 @@ struct BfTileArgs {
   const u32x4* PA;  // packed A-role operand of product 0
   const u32x4* PB;  // packed B-role operand of product 0
   int kbs;          // k blocks of 16 in both packed operands
+  int nblk_a;       // 32-row blocks held by PA / PB (rows padded to 32 only)
+  int nblk_b;
   int nprod;        // 2: second product uses (PB, PA)
   Epilogue ep;
 };
@@ -197,6 +211,10 @@ __device__ __forceinline__ void bf16_tile(const BfTileArgs& t, int tm, int tn, u
   const int KT = t.kbs / KB_PER_STAGE;
   const int steps = KT * t.nprod;
   const int64_t blk_stride = (int64_t)t.kbs * NHL * 64;
+  // Packed operands hold only the 32-row blocks that meet the matrix: a tile
+  // block past the last one reads the last block again.  Those rows feed only
+  // output rows / columns past M / N, which the epilogue never stores, so any
+  // finite data will do.
   // Global -> LDS without registers (global_load_lds_dwordx4): for one wave
   // and one i, the 64 lanes' chunks c = wave*64 + i*NT + lane are 64
   // consecutive chunks both in the packed operand (RUN is a multiple of 64)
@@ -206,6 +224,7 @@ __device__ __forceinline__ void bf16_tile(const BfTileArgs& t, int tm, int tn, u
     const int prod = s / KT, kt = s - prod * KT;
     const u32x4* A = prod ? t.PB : t.PA;
     const u32x4* B = prod ? t.PA : t.PB;
+    const int last_a = (prod ? t.nblk_b : t.nblk_a) - 1, last_b = (prod ? t.nblk_a : t.nblk_b) - 1;
     const int64_t koff = (int64_t)kt * RUN;
     u32x4* sa = lds + buf * (SA + SB);
     u32x4* sb = sa + SA;
@@ -213,13 +232,13 @@ __device__ __forceinline__ void bf16_tile(const BfTileArgs& t, int tm, int tn, u
     for(int i = 0; i < Cfg::PER_A; ++i) {
       const int c = tid + i * NT;
       const int blk = c / RUN, run = c - blk * RUN;
-      lds_dma16(A + (int64_t)(tm * (BM / 32) + blk) * blk_stride + koff + run, sa + wave_base + i * NT);
+      lds_dma16(A + (int64_t)min(tm * (BM / 32) + blk, last_a) * blk_stride + koff + run, sa + wave_base + i * NT);
     }
 #pragma unroll
     for(int i = 0; i < Cfg::PER_B; ++i) {
       const int c = tid + i * NT;
       const int blk = c / RUN, run = c - blk * RUN;
-      lds_dma16(B + (int64_t)(tn * (BN / 32) + blk) * blk_stride + koff + run, sb + wave_base + i * NT);
+      lds_dma16(B + (int64_t)min(tn * (BN / 32) + blk, last_b) * blk_stride + koff + run, sb + wave_base + i * NT);
     }
   };
   auto compute = [&](int cur) __attribute__((always_inline)) {
@@ -243,161 +262,6 @@ __device__ __forceinline__ void bf16_tile(const BfTileArgs& t, int tm, int tn, u
   static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
     constexpr int i = decltype(ij)::value / FN, j = decltype(ij)::value % FN;
     store_block32(t.ep, acc[i][j], tm * BM + wm * WM + i * 32, tn * BN + wn * WN + j * 32, lane);
-  });
-}
-
-// Stages one ROWS x 32 panel of an FP32 operand (rows row0.., k k0..) through
-// registers, splitting it on the way into the fragment-order LDS stage that
-// bf16_stage_mma reads.  Work unit = 8 consecutive k of one row (one chunk);
-// each of the NT threads owns UPT = ROWS*4/NT units:
-//   KMAJOR (element (r, k) at P[r*ld + k]): row tid % ROWS, UPT consecutive
-//       8-k segments -- one contiguous 32*UPT-byte read per thread;
-//   else (element (r, k) at P[k*ld + r]): UPT consecutive rows and one 8-k
-//       segment -- eight UPT-wide reads along the rows, consecutive threads
-//       on consecutive rows.
-// VEC: 128-bit (or UPT-wide) loads, which need aligned rows and a contiguous
-// extent that is a multiple of 4; otherwise bounds-checked scalar loads.
-template <int ROWS, int NT, int NHL, bool KMAJOR, bool VEC>
-struct SplitLoader {
-  static constexpr int UPT = ROWS * 4 / NT;
-  static_assert(UPT == 1 || UPT == 2 || UPT == 4, "unsupported panel shape");
-  static_assert(ROWS / UPT == 64 || KMAJOR, "one wave per 8-k segment");
-  float v[UPT][8];
-
-  __device__ __forceinline__ void load(const float* P, int64_t ld, int rows, int K, int row0, int k0, int tid) {
-    if constexpr(KMAJOR) {
-      const int r = tid % ROWS, seg0 = tid / ROWS * UPT;
-      const int gr = row0 + r;
-      const float* src = P + (int64_t)gr * ld;
-#pragma unroll
-      for(int u = 0; u < UPT; ++u) {
-        const int gk = k0 + (seg0 + u) * 8;
-        if constexpr(VEC) {
-          float4 x = make_float4(0.f, 0.f, 0.f, 0.f), y = x;
-          if(gr < rows && gk < K) x = *reinterpret_cast<const float4*>(src + gk);
-          if(gr < rows && gk + 4 < K) y = *reinterpret_cast<const float4*>(src + gk + 4);
-          v[u][0] = x.x; v[u][1] = x.y; v[u][2] = x.z; v[u][3] = x.w;
-          v[u][4] = y.x; v[u][5] = y.y; v[u][6] = y.z; v[u][7] = y.w;
-        } else {
-#pragma unroll
-          for(int e = 0; e < 8; ++e) v[u][e] = (gr < rows && gk + e < K) ? src[gk + e] : 0.0f;
-        }
-      }
-    } else {
-      const int r0 = (tid % 64) * UPT, seg = tid / 64;
-      const int gr = row0 + r0;
-#pragma unroll
-      for(int e = 0; e < 8; ++e) {
-        const int gk = k0 + seg * 8 + e;
-        const float* src = P + (int64_t)gk * ld + gr;
-        if constexpr(VEC && UPT == 4) {
-          float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
-          if(gk < K && gr < rows) x = *reinterpret_cast<const float4*>(src);
-          v[0][e] = x.x; v[1][e] = x.y; v[2][e] = x.z; v[3][e] = x.w;
-        } else if constexpr(VEC && UPT == 2) {
-          float2 x = make_float2(0.f, 0.f);
-          if(gk < K && gr < rows) x = *reinterpret_cast<const float2*>(src);
-          v[0][e] = x.x; v[1][e] = x.y;
-        } else {
-#pragma unroll
-          for(int u = 0; u < UPT; ++u) v[u][e] = (gk < K && gr + u < rows) ? src[u] : 0.0f;
-        }
-      }
-    }
-  }
-
-  // Splits the registers and writes them into the stage `st` (chunks
-  // [blk][kb][hl][lane] for this operand).
-  __device__ __forceinline__ void commit(u32x4* st, int tid) const {
-#pragma unroll
-    for(int u = 0; u < UPT; ++u) {
-      int r, seg;
-      if constexpr(KMAJOR) {
-        r = tid % ROWS;
-        seg = tid / ROWS * UPT + u;
-      } else {
-        r = (tid % 64) * UPT + u;
-        seg = tid / 64;
-      }
-      const int blk = r >> 5, kb = seg >> 1, lane = (r & 31) + 32 * (seg & 1);
-      u32x4 h, l;
-      split_chunk<NHL>(v[u], h, l);
-      u32x4* c = st + (blk * KB_PER_STAGE + kb) * NHL * 64 + lane;
-      c[0] = h;
-      if(NHL == 2) c[64] = l;
-    }
-  }
-};
-
-// Computes tile (tm, tn) of `op` straight from its FP32 operands, splitting
-// them into BF16 while staging (no packed copies), and writes it through
-// op.ep.  Same LDS size and MFMA sequence as bf16_tile, so the two give
-// bit-identical results.
-template <int BM, int BN, int WM, int WN, int NHL, bool AK, bool BKM, bool VEC>
-__device__ __forceinline__ void bf16_tile_direct(const DenseOp& op, int tm, int tn, u32x4* lds) {
-  using Cfg = BfCfg<BM, BN, WM, WN, NHL>;
-  constexpr int NT = Cfg::NT, FM = Cfg::FM, FN = Cfg::FN;
-  constexpr int SA = Cfg::CH_A, SB = Cfg::CH_B;
-  constexpr int BK = KB_PER_STAGE * 16;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int wm = wid / Cfg::NWN, wn = wid % Cfg::NWN;
-  const int m0 = tm * BM, n0 = tn * BN;
-
-  floatx16 acc[FM][FN];
-  zero_acc(acc);
-
-  SplitLoader<BM, NT, NHL, AK, VEC> la;
-  SplitLoader<BN, NT, NHL, BKM, VEC> lb;
-  const int KT = (op.K + BK - 1) / BK;
-  const int steps = KT * op.nprod;
-  auto issue = [&](int s) __attribute__((always_inline)) {
-    const int prod = s / KT, kt = s - prod * KT;
-    const float* pa = prod ? op.B : op.A;
-    const float* pb = prod ? op.A : op.B;
-    const int64_t lda = prod ? op.ldb : op.lda, ldb = prod ? op.lda : op.ldb;
-    la.load(pa, lda, op.M, op.K, m0, kt * BK, tid);
-    lb.load(pb, ldb, op.N, op.K, n0, kt * BK, tid);
-  };
-  auto commit = [&](int buf) __attribute__((always_inline)) {
-    u32x4* sa = lds + buf * (SA + SB);
-    la.commit(sa, tid);
-    lb.commit(sa + SA, tid);
-  };
-  auto compute = [&](int cur) __attribute__((always_inline)) {
-    const u32x4* sa = lds + cur * (SA + SB);
-    bf16_stage_mma<FM, FN, NHL>(acc, sa, sa + SA, wm, wn, lane);
-  };
-
-  // Two-deep pipeline, one barrier per step.  In step s the wave contracts
-  // the first k block of stage s, then splits the registers loaded one step
-  // earlier (stage s+1) into the other LDS buffer and issues the loads of
-  // stage s+2 into the freed registers, then contracts the second k block:
-  // the split runs while the MFMAs of stage s are still in the pipe and the
-  // loads have a whole step to arrive.  Loads past the last stage repeat the
-  // last stage (in bounds, never read), so the loop has no branches.
-  auto compute_kb = [&](int cur, int kb) __attribute__((always_inline)) {
-    const u32x4* sa = lds + cur * (SA + SB);
-    bf16_kb_mma<FM, FN, NHL>(acc, sa, sa + SA, wm, wn, lane, kb);
-  };
-  if(steps > 0) {
-    issue(0);
-    commit(0);
-    issue(steps > 1 ? 1 : 0);
-  }
-  __syncthreads();
-  for(int s = 0; s + 1 < steps; ++s) {
-    compute_kb(s & 1, 0);
-    commit((s + 1) & 1);
-    issue(s + 2 < steps ? s + 2 : steps - 1);
-#pragma unroll
-    for(int kb = 1; kb < KB_PER_STAGE; ++kb) compute_kb(s & 1, kb);
-    __syncthreads();
-  }
-  if(steps > 0) compute((steps - 1) & 1);
-
-  static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
-    constexpr int i = decltype(ij)::value / FN, j = decltype(ij)::value % FN;
-    store_block32(op.ep, acc[i][j], m0 + wm * WM + i * 32, n0 + wn * WN + j * 32, lane);
   });
 }
 
