@@ -99,7 +99,7 @@ def grouped_workload(torch, mfx, h, steps, warmup, seed=1):
 
     log("grouped: autotuning tile policy ...")
     policy = h.autotune_grouped(probs, deps, prec="split-bf16")
-    big, small, bound = mfx.grouped_policy_info(policy)
+    big, small, bound, per_level = mfx.grouped_policy_info(policy)
     graph = h.graph_create(probs, deps, prec="split-bf16")
     for _ in range(warmup):
         graph.launch()
@@ -147,7 +147,8 @@ def grouped_workload(torch, mfx, h, steps, warmup, seed=1):
             "flop": flop, "ms": round(ms, 4), "tflops": round(flop / (ms / 1e3) / 1e12, 2),
             "per_problem_launches_ms": round(t_single, 4),
             "per_problem_launches_tflops": round(flop / (t_single / 1e3) / 1e12, 2),
-            "tile_policy": {"id": policy, "large_config": big, "small_config": small, "waste_bound": bound},
+            "tile_policy": {"id": policy, "large_config": big, "small_config": small, "waste_bound": bound,
+                            "launch_per_level": per_level},
             "chain_rel_err": chain_err, "sampled_rel_err_max": worst}
 
 

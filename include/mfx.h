@@ -242,12 +242,16 @@ mfx_status mfx_autotune_grouped(mfx_handle handle, mfx_precision prec,
  * Grouped tile policies: a launch runs one policy, a pair of tile
  * configurations (ids as in mfx_config_info); each problem takes the large
  * one unless that pads its output area by more than `waste_bound` (0: always
- * the large one), else the small one.  FP32 launches run configuration 3 in
- * place of 0.
+ * the large one), else the small one, and on a dependency level whose large
+ * tiles would not give every CU one, the small one.  FP32 launches run
+ * configuration 3 in place of 0.  *per_level = 1: the launch is split into
+ * one pre-pass + persistent kernel per dependency level (kernel boundaries
+ * order the levels; no in-kernel waits), else one persistent kernel with
+ * in-kernel dependency waits.
  */
 int32_t mfx_grouped_policy_count(void);
 mfx_status mfx_grouped_policy_info(int32_t policy, int32_t* large_config, int32_t* small_config,
-    double* waste_bound);
+    double* waste_bound, int32_t* per_level);
 /* Forces `policy` (or -1 for the tuned / estimated choice) for later grouped launches. */
 mfx_status mfx_force_grouped_policy(mfx_handle handle, int32_t policy);
 /* Saves / loads the tuning table as JSON. */

@@ -24,7 +24,7 @@ std::map<std::string, int> tuned_snapshot();
 void replace_tuned(const std::map<std::string, int>& t);
 mfx_status dispatch_dense(Handle* h, mfx_precision prec, DenseOp& op, int config, hipStream_t s);
 int grouped_policy_count();
-bool grouped_policy_info(int id, int* big, int* small, double* bound);
+bool grouped_policy_info(int id, int* big, int* small, double* bound, int* per_level);
 std::string grouped_tune_key(mfx_precision p, const mfx_gemm_problem* problems, int count, const mfx_dep* deps,
     int ndeps);
 mfx_status run_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* problems, int count,
@@ -155,18 +155,19 @@ mfx_status mfx_autotune_grouped(mfx_handle handle, mfx_precision prec, const mfx
     // FP32 runs 256x128 for the 256x256 configuration: time each distinct
     // pair of configurations once.
     auto pair_of = [&](int q) {
-      int big, small;
+      int big, small, per_level;
       double bound;
-      grouped_policy_info(q, &big, &small, &bound);
+      grouped_policy_info(q, &big, &small, &bound, &per_level);
       if(prec == MFX_PREC_FP32) {
         big = big == 0 ? 3 : big;
         small = small == 0 ? 3 : small;
       }
-      return std::make_tuple(big, small, big == small ? 0.0 : bound);
+      return std::make_tuple(big, small, big == small ? 0.0 : bound, per_level);
     };
     bool dup = false;
     for(int q = 0; q < p; ++q) dup = dup || pair_of(q) == pair_of(p);
-    if(dup) continue;
+    // without dependencies the per-level form is the same single launch
+    if(dup || (ndeps == 0 && std::get<3>(pair_of(p)))) continue;
     h->grouped_policy = p;
     double ms = 0.0;
     st = time_ms(s, 3, [&] { return run_grouped(h, prec, problems, count, deps, ndeps, s); }, &ms);
@@ -193,14 +194,15 @@ mfx_status mfx_force_grouped_policy(mfx_handle handle, int32_t policy) {
 }
 
 mfx_status mfx_grouped_policy_info(int32_t policy, int32_t* large_config, int32_t* small_config,
-    double* waste_bound) {
-  if(!large_config || !small_config || !waste_bound) return invalid("null output pointer");
-  int big, small;
+    double* waste_bound, int32_t* per_level) {
+  if(!large_config || !small_config || !waste_bound || !per_level) return invalid("null output pointer");
+  int big, small, lv;
   double bound;
-  if(!grouped_policy_info(policy, &big, &small, &bound)) return invalid("grouped policy out of range");
+  if(!grouped_policy_info(policy, &big, &small, &bound, &lv)) return invalid("grouped policy out of range");
   *large_config = big;
   *small_config = small;
   *waste_bound = bound;
+  *per_level = lv;
   return MFX_SUCCESS;
 }
 

@@ -62,6 +62,8 @@ constexpr BfConfig kBfConfigs[] = {
     {64, 64, 4},    // 2: 32x32 per wave
     {256, 128, 4},  // 3: 128x64 per wave
     {256, 256, 8},  // 4: 128x64 per wave, two waves per SIMD
+    {256, 128, 8},  // 5: 64x64 per wave, two waves per SIMD
+    {128, 128, 8},  // 6: 64x32 per wave, two waves per SIMD
 };
 constexpr int kBfConfigCount = sizeof(kBfConfigs) / sizeof(kBfConfigs[0]);
 
@@ -186,6 +188,8 @@ mfx_status launch_dense_bf16x(Handle* h, const DenseOp& op, bool split, int conf
     case 2: return launch_bf_cfg<64, 64, 32, 32>(h, p, split, grid, s);
     case 3: return launch_bf_cfg<256, 128, 128, 64>(h, p, split, grid, s);
     case 4: return launch_bf_cfg<256, 256, 128, 64>(h, p, split, grid, s);
+    case 5: return launch_bf_cfg<256, 128, 64, 64>(h, p, split, grid, s);
+    case 6: return launch_bf_cfg<128, 128, 64, 32>(h, p, split, grid, s);
   }
   return invalid("unknown BF16 configuration");
 }

@@ -42,6 +42,8 @@ constexpr F32Config kF32Configs[] = {
     {64, 64, 32, 4},    // 2: 32x32 per wave, small problems
     {256, 128, 16, 4},  // 3: 128x64 per wave
     {256, 256, 16, 8},  // 4: 128x64 per wave, two waves per SIMD
+    {256, 128, 16, 8},  // 5: 64x64 per wave, two waves per SIMD
+    {128, 128, 16, 8},  // 6: 64x32 per wave, two waves per SIMD
 };
 constexpr int kF32ConfigCount = sizeof(kF32Configs) / sizeof(kF32Configs[0]);
 
@@ -106,6 +108,8 @@ mfx_status launch_dense_f32(Handle* h, const DenseOp& op, int config, hipStream_
     case 2: return launch_cfg<64, 64, 32, 32, 32>(h, p, grid, s);
     case 3: return launch_cfg<256, 128, 16, 128, 64>(h, p, grid, s);
     case 4: return launch_cfg<256, 256, 16, 128, 64>(h, p, grid, s);
+    case 5: return launch_cfg<256, 128, 16, 64, 64>(h, p, grid, s);
+    case 6: return launch_cfg<128, 128, 16, 64, 32>(h, p, grid, s);
   }
   return invalid("unknown FP32 configuration");
 }

@@ -93,16 +93,28 @@ struct GroupedArgs {
 };
 
 // Tile geometry of configuration C (the same ids as the single-problem
-// kernels): BM x BN per workgroup of four waves, WM x WN per wave; BK is the
-// FP32 k-step.
+// kernels): BM x BN per workgroup, WM x WN per wave (four waves for 0-3,
+// eight, two per SIMD, for 4-6); BK is the FP32 k-step.
 template <int C> struct Geo;
 template <> struct Geo<0> { static constexpr int BM = 256, BN = 256, WM = 128, WN = 128, BK = 16; };
 template <> struct Geo<1> { static constexpr int BM = 128, BN = 128, WM = 64, WN = 64, BK = 16; };
 template <> struct Geo<2> { static constexpr int BM = 64, BN = 64, WM = 32, WN = 32, BK = 32; };
 template <> struct Geo<3> { static constexpr int BM = 256, BN = 128, WM = 128, WN = 64, BK = 16; };
+template <> struct Geo<4> { static constexpr int BM = 256, BN = 256, WM = 128, WN = 64, BK = 16; };
+template <> struct Geo<5> { static constexpr int BM = 256, BN = 128, WM = 64, WN = 64, BK = 16; };
+template <> struct Geo<6> { static constexpr int BM = 128, BN = 128, WM = 64, WN = 32, BK = 16; };
+// threads of a workgroup running configuration C
+template <int C>
+constexpr int geo_threads() {
+  return (Geo<C>::BM / Geo<C>::WM) * (Geo<C>::BN / Geo<C>::WN) * 64;
+}
 
-__host__ __device__ inline int cfg_bm(int config) { return config == 1 ? 128 : config == 2 ? 64 : 256; }
-__host__ __device__ inline int cfg_bn(int config) { return config == 0 ? 256 : config == 2 ? 64 : 128; }
+__host__ __device__ inline int cfg_bm(int config) { return config == 1 || config == 6 ? 128 : config == 2 ? 64 : 256; }
+__host__ __device__ inline int cfg_bn(int config) {
+  return config == 0 || config == 4 ? 256 : config == 2 ? 64 : 128;
+}
+// four waves per workgroup, or eight for configurations 4-6
+__host__ __device__ inline int cfg_threads(int config) { return config >= 4 ? 512 : 256; }
 
 // Cross-XCD ordering costs cache maintenance on gfx950: an agent-scope
 // release writes back the XCD's L2 (buffer_wbl2) and an agent-scope acquire

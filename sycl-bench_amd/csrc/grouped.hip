@@ -22,6 +22,7 @@ namespace {
 struct GroupedPolicy {
   int big, small;
   double bound;
+  int per_level;  // 1: one pre-pass + persistent launch per dependency level, no in-kernel waits
 };
 constexpr GroupedPolicy kPolicies[] = {
     {0, 0, 0.0},   // 0
@@ -34,13 +35,24 @@ constexpr GroupedPolicy kPolicies[] = {
     {1, 1, 0.0},   // 7
     {1, 2, 1.34},  // 8
     {2, 2, 0.0},   // 9
+    {4, 4, 0.0},   // 10: eight-wave tiles from here on
+    {4, 6, 1.34},  // 11
+    {4, 5, 1.34},  // 12
+    {5, 5, 0.0},   // 13
+    {5, 6, 1.34},  // 14
+    {6, 6, 0.0},   // 15
+    {4, 4, 0.0, 1},   // 16: one launch per dependency level from here on
+    {4, 6, 1.34, 1},  // 17
+    {4, 5, 1.34, 1},  // 18
+    {3, 1, 1.34, 1},  // 19
+    {6, 6, 0.0, 1},   // 20
 };
 constexpr int kPolicyCount = sizeof(kPolicies) / sizeof(kPolicies[0]);
 
 // Relative per-CU contraction rate of each tile configuration on large
 // problems (larger tiles reuse each loaded panel more), used by the
 // untuned policy estimate.
-constexpr double kTileRate[4] = {1.0, 0.6, 0.2, 0.85};
+constexpr double kTileRate[7] = {1.0, 0.6, 0.2, 0.85, 1.15, 0.95, 0.65};
 
 // The configuration a grouped launch of `prec` runs for configuration c.
 int grouped_config(mfx_precision prec, int c) { return prec == MFX_PREC_FP32 && c == 0 ? 3 : c; }
@@ -62,6 +74,26 @@ std::vector<int> problem_levels(int count, const mfx_dep* deps, int ndeps) {
   return level;
 }
 
+// Tile configuration of every problem under a policy: the padding rule,
+// then, on a dependency level whose tiles would not give every CU one, the
+// small tile for all of the level's problems (a chain's later levels hold
+// few problems, and a level cannot overlap the next).
+std::vector<int> assign_configs(const GroupedPolicy& pol, mfx_precision prec, const mfx_gemm_problem* problems,
+    int count, const std::vector<int>& level, int cus) {
+  std::vector<int> cfg(count);
+  for(int i = 0; i < count; ++i) cfg[i] = problem_config(pol, prec, problems[i].m, problems[i].n);
+  const int small = grouped_config(prec, pol.small);
+  if(grouped_config(prec, pol.big) == small) return cfg;
+  const int levels = count ? *std::max_element(level.begin(), level.end()) + 1 : 0;
+  std::vector<int64_t> tiles(levels, 0);
+  for(int i = 0; i < count; ++i)
+    if(problems[i].m > 0 && problems[i].n > 0)
+      tiles[level[i]] += ceil_div(problems[i].m, cfg_bm(cfg[i])) * ceil_div(problems[i].n, cfg_bn(cfg[i]));
+  for(int i = 0; i < count; ++i)
+    if(level[i] > 0 && tiles[level[i]] < cus) cfg[i] = small;
+  return cfg;
+}
+
 // Estimated duration (arbitrary units) of a launch under a policy: per level,
 // the padded tile work spread over the CUs plus half a tile of tail.
 double policy_estimate(const GroupedPolicy& pol, mfx_precision prec, const mfx_gemm_problem* problems, int count,
@@ -69,10 +101,11 @@ double policy_estimate(const GroupedPolicy& pol, mfx_precision prec, const mfx_g
   const int levels = count ? *std::max_element(level.begin(), level.end()) + 1 : 0;
   std::vector<double> work(levels, 0.0), tmax(levels, 0.0);
   std::vector<int64_t> ntiles(levels, 0);
+  const std::vector<int> cfg = assign_configs(pol, prec, problems, count, level, cus);
   for(int i = 0; i < count; ++i) {
     const auto& q = problems[i];
     if(q.m <= 0 || q.n <= 0) continue;
-    const int c = problem_config(pol, prec, q.m, q.n);
+    const int c = cfg[i];
     const int64_t tiles = ceil_div(q.m, cfg_bm(c)) * ceil_div(q.n, cfg_bn(c));
     const double t = (double)cfg_bm(c) * cfg_bn(c) * (double)round_up(std::max<int64_t>(q.k, 1), 32) / kTileRate[c];
     work[level[i]] += t * tiles;
@@ -91,11 +124,12 @@ double policy_estimate(const GroupedPolicy& pol, mfx_precision prec, const mfx_g
 
 int grouped_policy_count() { return kPolicyCount; }
 
-bool grouped_policy_info(int id, int* big, int* small, double* bound) {
+bool grouped_policy_info(int id, int* big, int* small, double* bound, int* per_level) {
   if(id < 0 || id >= kPolicyCount) return false;
   *big = kPolicies[id].big;
   *small = kPolicies[id].small;
   *bound = kPolicies[id].bound;
+  *per_level = kPolicies[id].per_level;
   return true;
 }
 
@@ -117,17 +151,21 @@ int choose_grouped_policy(Handle* h, mfx_precision prec, const mfx_gemm_problem*
   if(h->grouped_policy >= 0) return h->grouped_policy;
   int forced = h->forced_config[prec];
   if(forced >= 0) {
-    if(forced == 4) forced = 0;  // the grouped kernels run four waves
     for(int i = 0; i < kPolicyCount; ++i)
-      if(kPolicies[i].big == forced && kPolicies[i].small == forced) return i;
+      if(kPolicies[i].big == forced && kPolicies[i].small == forced && !kPolicies[i].per_level) return i;
   }
   const int tuned = tuned_lookup(grouped_tune_key(prec, problems, count, deps, ndeps));
   if(tuned >= 0 && tuned < kPolicyCount) return tuned;
   const auto level = problem_levels(count, deps, ndeps);
   const int cus = h->cu_count > 0 ? h->cu_count : 256;
+  const bool levels = count > 0 && *std::max_element(level.begin(), level.end()) > 0;
   int best = 0;
   double best_est = -1.0;
   for(int i = 0; i < kPolicyCount; ++i) {
+    // With dependency levels, one launch per level (measured faster than
+    // in-kernel waits, whose agent-scope fences write back / invalidate L2);
+    // without, the two modes are the same launch.
+    if(levels != (kPolicies[i].per_level != 0)) continue;
     const double e = policy_estimate(kPolicies[i], prec, problems, count, level, cus);
     if(best_est < 0 || e < best_est) {
       best_est = e;
@@ -165,7 +203,11 @@ struct GroupedPlan {
   size_t off_problems = 0, off_items = 0, off_jobs = 0, off_counters = 0, off_packed = 0;
   std::vector<std::pair<int, int>> job_list;  // (problem, 1 = A / 0 = B) packed by the pre-pass
   std::vector<PackJob> jobs;                  // filled in bind_grouped
-  int pack_blocks = 0;                        // grid of the pre-pass
+  // Launch segments: one for the whole queue, or one per dependency level
+  // (per_level).  Segment l covers items [seg_items[l], seg_items[l+1]) and
+  // pre-pass jobs [seg_jobs[l], seg_jobs[l+1]) in seg_blocks[l] blocks.
+  bool per_level = false;
+  std::vector<int> seg_items, seg_jobs, seg_blocks;
   const PackJob* dev_jobs = nullptr;          // the job table in device memory
   size_t total_bytes = 0;
   int policy = 0, big = 0, small = 0;  // tile policy and its configurations
@@ -227,10 +269,16 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
   const GroupedPolicy& pol = kPolicies[pl.policy];
   pl.big = grouped_config(prec, pol.big);
   pl.small = grouped_config(prec, pol.small);
+  const std::vector<int> cfg =
+      assign_configs(pol, prec, problems, count, level, h->cu_count > 0 ? h->cu_count : 256);
   const bool packed = prec != MFX_PREC_FP32;
   const int nhl = prec == MFX_PREC_SPLIT_BF16 ? 2 : 1;
   size_t packed_bytes = 0;
   int panels = 0;
+  const int max_level = count ? *std::max_element(level.begin(), level.end()) : 0;
+  // One launch per level: kernel boundaries order the levels, so no problem
+  // waits or publishes in-kernel and every operand is packed by a pre-pass.
+  pl.per_level = pol.per_level && max_level > 0;
 
   for(int i = 0; i < count; ++i) {
     const auto& q = problems[i];
@@ -248,7 +296,7 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
     P.ldc = q.ldc;
     P.a_kmajor = q.transA == 0;
     P.b_kmajor = q.transB == 1;
-    P.config = problem_config(pol, prec, q.m, q.n);
+    P.config = cfg[i];
     const int bm = cfg_bm(P.config), bn = cfg_bn(P.config);
     P.tiles_m = (int)ceil_div(q.m, bm);
     P.tiles_n = (int)ceil_div(q.n, bn);
@@ -268,13 +316,9 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
       P.kbs = (int)(round_up(q.k, 16 * KB_PER_STAGE) / 16);
       P.pack_slices = (int)ceil_div(P.kbs, PACK_KBLOCKS);
       packed_bytes += packed_sizes(P, nhl).first + packed_sizes(P, nhl).second;
-      if(P.ndeps == 0) {
-        // inputs only: packed by the pre-pass kernel before the tiles start
+      if(P.ndeps == 0 || pl.per_level) {
+        // packed by a pre-pass kernel before the tiles start (jobs listed below)
         P.prepacked = 1;
-        if(P.kbs > 0) {
-          pl.job_list.push_back({i, 1});
-          pl.job_list.push_back({i, 0});
-        }
       } else {
         P.pack_a_ctr = panels;
         panels += P.tiles_m;
@@ -287,13 +331,19 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
 
   // Items level by level: the level's pack items (largest K first, so the
   // first tiles' panels are ready first), then its tiles, largest K first, a
-  // problem's tiles row by row.
-  const int max_level = count ? *std::max_element(level.begin(), level.end()) : 0;
+  // problem's tiles row by row.  Pre-pass jobs level by level too.
+  pl.seg_items.assign(1, 0);
+  pl.seg_jobs.assign(1, 0);
   for(int lv = 0; lv <= max_level; ++lv) {
     std::vector<int> at;
     for(int i = 0; i < count; ++i)
       if(level[i] == lv && pl.problems[i].tiles_m * pl.problems[i].tiles_n > 0) at.push_back(i);
     std::stable_sort(at.begin(), at.end(), [&](int a, int b) { return pl.problems[a].K > pl.problems[b].K; });
+    for(int i : at)
+      if(pl.problems[i].prepacked && pl.problems[i].kbs > 0) {
+        pl.job_list.push_back({i, 1});
+        pl.job_list.push_back({i, 0});
+      }
     if(packed)
       for(int i : at) {
         const GProblem& P = pl.problems[i];
@@ -306,13 +356,25 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
     for(int i : at)
       for(int tm = 0; tm < pl.problems[i].tiles_m; ++tm)
         for(int tn = 0; tn < pl.problems[i].tiles_n; ++tn) pl.items.push_back({ITEM_TILE, i, tm, tn});
+    if(pl.per_level) {
+      pl.seg_items.push_back((int)pl.items.size());
+      pl.seg_jobs.push_back((int)pl.job_list.size());
+    }
   }
+  if(!pl.per_level) {
+    pl.seg_items.push_back((int)pl.items.size());
+    pl.seg_jobs.push_back((int)pl.job_list.size());
+  } else {
+    for(auto& P : pl.problems) P.ndeps = P.consumed = 0;  // ordered by the launches
+  }
+  const int segments = (int)pl.seg_items.size() - 1;
 
   pl.off_problems = 0;
   pl.off_items = round_up(sizeof(GProblem) * std::max(1, count), 256);
   pl.off_jobs = pl.off_items + round_up(sizeof(GItem) * std::max<size_t>(1, pl.items.size()), 256);
   pl.off_counters = pl.off_jobs + round_up(sizeof(PackJob) * std::max<size_t>(1, pl.job_list.size()), 256);
-  pl.counter_ints = 2 + (size_t)count + (size_t)panels;
+  // counters: error flag, one queue head per segment, per-problem done, per-panel pack counts
+  pl.counter_ints = 1 + (size_t)segments + (size_t)count + (size_t)panels;
   pl.off_packed = pl.off_counters + round_up(pl.counter_ints * sizeof(int), 256);
   pl.total_bytes = pl.off_packed + packed_bytes;
   return MFX_SUCCESS;
@@ -335,8 +397,14 @@ static mfx_status bind_grouped(Handle* h, GroupedPlan& pl, void* ws, GroupedArgs
       off += sz.second;
     }
     pl.jobs.clear();
-    int blocks = 0;
-    for(const auto& jb : pl.job_list) {
+    pl.seg_blocks.assign(pl.seg_items.size() - 1, 0);
+    int blocks = 0, seg = 0;
+    for(size_t ji = 0; ji < pl.job_list.size(); ++ji) {
+      while((int)ji >= pl.seg_jobs[seg + 1]) {
+        pl.seg_blocks[seg++] = blocks;
+        blocks = 0;
+      }
+      const auto& jb = pl.job_list[ji];
       const GProblem& P = pl.problems[jb.first];
       const bool is_a = jb.second == 1;
       PackJob j;
@@ -350,7 +418,7 @@ static mfx_status bind_grouped(Handle* h, GroupedPlan& pl, void* ws, GroupedArgs
       blocks += j.blocks;
       pl.jobs.push_back(j);
     }
-    pl.pack_blocks = blocks;
+    if(!pl.job_list.empty()) pl.seg_blocks[seg] = blocks;
   }
   // The previous upload from the staging buffer must have completed.
   if(h->staging_ready) MFX_HIP(hipEventSynchronize(h->staging_ready));
@@ -373,27 +441,39 @@ static mfx_status bind_grouped(Handle* h, GroupedPlan& pl, void* ws, GroupedArgs
   g.items = reinterpret_cast<const GItem*>(base + pl.off_items);
   pl.dev_jobs = reinterpret_cast<const PackJob*>(base + pl.off_jobs);
   g.nitems = (int)pl.items.size();
-  g.head = counters;
-  g.error = counters + 1;
-  g.done = counters + 2;
-  g.pack_ctr = counters + 2 + pl.problems.size();
+  const int segments = (int)pl.seg_items.size() - 1;
+  g.error = counters;
+  g.head = counters + 1;  // segment 0; segment l uses head + l
+  g.done = counters + 1 + segments;
+  g.pack_ctr = counters + 1 + segments + pl.problems.size();
   g.spin_limit = h->spin_limit;
   return MFX_SUCCESS;
 }
 
-// Enqueues the counter reset and the persistent kernel.
+// Enqueues the counter reset, then per segment the pre-pass and the
+// persistent kernel.
 static mfx_status launch_grouped_kernels(Handle* h, const GroupedPlan& pl, const GroupedArgs& g, hipStream_t s) {
   const int n = (int)pl.counter_ints;
   MFX_CHECK(timed_launch(h, "grouped_reset", s, [&] {
-    zero_ints_kernel<<<(int)std::min<int64_t>(ceil_div(n, 256), 1024), 256, 0, s>>>(g.head, n, g.error);
+    zero_ints_kernel<<<(int)std::min<int64_t>(ceil_div(n, 256), 1024), 256, 0, s>>>(g.head - 1, n, g.error);
   }));
   if(pl.items.empty()) return MFX_SUCCESS;
-  if(pl.prec != MFX_PREC_FP32)
-    MFX_CHECK(launch_pack_jobs(h, pl.prec == MFX_PREC_SPLIT_BF16 ? 2 : 1, pl.dev_jobs, (int)pl.jobs.size(),
-        pl.pack_blocks, s));
   const int grid = h->cu_count > 0 ? h->cu_count : 256;
-  if(pl.prec == MFX_PREC_FP32) return launch_grouped_f32(h, pl.big, pl.small, pl.all_vec, g, grid, s);
-  return launch_grouped_bf16(h, pl.prec == MFX_PREC_SPLIT_BF16 ? 2 : 1, pl.big, pl.small, pl.all_vec, g, grid, s);
+  const int nhl = pl.prec == MFX_PREC_SPLIT_BF16 ? 2 : 1;
+  const int segments = (int)pl.seg_items.size() - 1;
+  for(int l = 0; l < segments; ++l) {
+    if(pl.prec != MFX_PREC_FP32 && !pl.seg_blocks.empty())
+      MFX_CHECK(launch_pack_jobs(h, nhl, pl.dev_jobs + pl.seg_jobs[l], pl.seg_jobs[l + 1] - pl.seg_jobs[l],
+          pl.seg_blocks[l], s));
+    GroupedArgs gl = g;
+    gl.items = g.items + pl.seg_items[l];
+    gl.nitems = pl.seg_items[l + 1] - pl.seg_items[l];
+    gl.head = g.head + l;
+    if(gl.nitems == 0) continue;
+    if(pl.prec == MFX_PREC_FP32) MFX_CHECK(launch_grouped_f32(h, pl.big, pl.small, pl.all_vec, gl, grid, s));
+    else MFX_CHECK(launch_grouped_bf16(h, nhl, pl.big, pl.small, pl.all_vec, gl, grid, s));
+  }
+  return MFX_SUCCESS;
 }
 
 mfx_status run_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* problems, int count,

@@ -13,7 +13,7 @@ namespace {
 // One pack item: all loads of a thread are issued before any of its stores
 // (the stores could alias the loads as far as the compiler knows, so it
 // would not reorder them itself).
-template <int NHL>
+template <int NHL, int NT>
 __device__ void pack_item(const GProblem& P, bool is_a, int blk, int slice) {
   PackSrc s;
   s.src = is_a ? P.A : P.B;
@@ -25,16 +25,16 @@ __device__ void pack_item(const GProblem& P, bool is_a, int blk, int slice) {
   u32x4* dst = is_a ? P.packed_a : P.packed_b;
   const int kb0 = slice * PACK_KBLOCKS;
   const int nkb = min(PACK_KBLOCKS, P.kbs - kb0);
-  constexpr int PER = PACK_KBLOCKS * 64 / 256;
+  constexpr int PER = PACK_KBLOCKS * 64 / NT;
   float v[PER][8];
 #pragma unroll
   for(int i = 0; i < PER; ++i) {
-    const int c = threadIdx.x + i * 256;
+    const int c = threadIdx.x + i * NT;
     if((c >> 6) < nkb) load_chunk8(s, blk, kb0 + (c >> 6), c & 63, v[i]);
   }
 #pragma unroll
   for(int i = 0; i < PER; ++i) {
-    const int c = threadIdx.x + i * 256;
+    const int c = threadIdx.x + i * NT;
     if((c >> 6) < nkb) store_chunk<NHL>(dst, blk, kb0 + (c >> 6), c & 63, P.kbs, v[i]);
   }
 }
@@ -68,7 +68,9 @@ constexpr int bf16_smem_chunks() {
 }
 
 template <int NHL, int BIG, int SMALL>
-__global__ __launch_bounds__(256) void grouped_bf16x_kernel(const GroupedArgs g) {
+__global__ __launch_bounds__(geo_threads<BIG>()) void grouped_bf16x_kernel(const GroupedArgs g) {
+  constexpr int NT = geo_threads<BIG>();
+  static_assert(geo_threads<SMALL>() == NT, "both tiles of a policy use the same workgroup");
   constexpr int S0 = bf16_smem_chunks<NHL, BIG>(), S1 = bf16_smem_chunks<NHL, SMALL>();
   __shared__ u32x4 lds[S0 > S1 ? S0 : S1];
   __shared__ int slot;
@@ -81,7 +83,7 @@ __global__ __launch_bounds__(256) void grouped_bf16x_kernel(const GroupedArgs g)
     if(item.type != ITEM_TILE) {
       const bool is_a = item.type == ITEM_PACK_A;
       if(!wait_producers(P, g)) continue;
-      pack_item<NHL>(P, is_a, item.x, item.y);
+      pack_item<NHL, NT>(P, is_a, item.x, item.y);
       const int panel = item.x * 32 / (is_a ? bm : bn);
       signal(&g.pack_ctr[(is_a ? P.pack_a_ctr : P.pack_b_ctr) + panel]);
       continue;
@@ -113,18 +115,29 @@ __global__ __launch_bounds__(256) void grouped_bf16x_kernel(const GroupedArgs g)
   }
 }
 
+template <int NHL, int BIG, int SMALL>
+void launch_one(const GroupedArgs& g, int grid, hipStream_t s) {
+  grouped_bf16x_kernel<NHL, BIG, SMALL><<<grid, geo_threads<BIG>(), 0, s>>>(g);
+}
+
 template <int NHL>
 bool launch_pair(int big, int small, const GroupedArgs& g, int grid, hipStream_t s) {
-  switch(big * 4 + small) {
-    case 0 * 4 + 0: grouped_bf16x_kernel<NHL, 0, 0><<<grid, 256, 0, s>>>(g); return true;
-    case 0 * 4 + 1: grouped_bf16x_kernel<NHL, 0, 1><<<grid, 256, 0, s>>>(g); return true;
-    case 0 * 4 + 2: grouped_bf16x_kernel<NHL, 0, 2><<<grid, 256, 0, s>>>(g); return true;
-    case 3 * 4 + 3: grouped_bf16x_kernel<NHL, 3, 3><<<grid, 256, 0, s>>>(g); return true;
-    case 3 * 4 + 1: grouped_bf16x_kernel<NHL, 3, 1><<<grid, 256, 0, s>>>(g); return true;
-    case 3 * 4 + 2: grouped_bf16x_kernel<NHL, 3, 2><<<grid, 256, 0, s>>>(g); return true;
-    case 1 * 4 + 1: grouped_bf16x_kernel<NHL, 1, 1><<<grid, 256, 0, s>>>(g); return true;
-    case 1 * 4 + 2: grouped_bf16x_kernel<NHL, 1, 2><<<grid, 256, 0, s>>>(g); return true;
-    case 2 * 4 + 2: grouped_bf16x_kernel<NHL, 2, 2><<<grid, 256, 0, s>>>(g); return true;
+  switch(big * 8 + small) {
+    case 0 * 8 + 0: launch_one<NHL, 0, 0>(g, grid, s); return true;
+    case 0 * 8 + 1: launch_one<NHL, 0, 1>(g, grid, s); return true;
+    case 0 * 8 + 2: launch_one<NHL, 0, 2>(g, grid, s); return true;
+    case 3 * 8 + 3: launch_one<NHL, 3, 3>(g, grid, s); return true;
+    case 3 * 8 + 1: launch_one<NHL, 3, 1>(g, grid, s); return true;
+    case 3 * 8 + 2: launch_one<NHL, 3, 2>(g, grid, s); return true;
+    case 1 * 8 + 1: launch_one<NHL, 1, 1>(g, grid, s); return true;
+    case 1 * 8 + 2: launch_one<NHL, 1, 2>(g, grid, s); return true;
+    case 2 * 8 + 2: launch_one<NHL, 2, 2>(g, grid, s); return true;
+    case 4 * 8 + 4: launch_one<NHL, 4, 4>(g, grid, s); return true;
+    case 4 * 8 + 5: launch_one<NHL, 4, 5>(g, grid, s); return true;
+    case 4 * 8 + 6: launch_one<NHL, 4, 6>(g, grid, s); return true;
+    case 5 * 8 + 5: launch_one<NHL, 5, 5>(g, grid, s); return true;
+    case 5 * 8 + 6: launch_one<NHL, 5, 6>(g, grid, s); return true;
+    case 6 * 8 + 6: launch_one<NHL, 6, 6>(g, grid, s); return true;
   }
   return false;
 }

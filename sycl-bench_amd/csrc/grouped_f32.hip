@@ -23,7 +23,8 @@ constexpr int f32_smem_floats() {
 
 // VEC: every problem of the launch allows 128-bit operand loads.
 template <int BIG, int SMALL, bool VEC>
-__global__ __launch_bounds__(256) void grouped_f32_kernel(const GroupedArgs g) {
+__global__ __launch_bounds__(geo_threads<BIG>()) void grouped_f32_kernel(const GroupedArgs g) {
+  static_assert(geo_threads<SMALL>() == geo_threads<BIG>(), "both tiles of a policy use the same workgroup");
   constexpr int S0 = f32_smem_floats<BIG>(), S1 = f32_smem_floats<SMALL>();
   __shared__ float smem[S0 > S1 ? S0 : S1];
   __shared__ int slot;
@@ -56,15 +57,26 @@ __global__ __launch_bounds__(256) void grouped_f32_kernel(const GroupedArgs g) {
   }
 }
 
+template <int BIG, int SMALL, bool VEC>
+void launch_one(const GroupedArgs& g, int grid, hipStream_t s) {
+  grouped_f32_kernel<BIG, SMALL, VEC><<<grid, geo_threads<BIG>(), 0, s>>>(g);
+}
+
 template <bool VEC>
 bool launch_pair(int big, int small, const GroupedArgs& g, int grid, hipStream_t s) {
-  switch(big * 4 + small) {
-    case 3 * 4 + 3: grouped_f32_kernel<3, 3, VEC><<<grid, 256, 0, s>>>(g); return true;
-    case 3 * 4 + 1: grouped_f32_kernel<3, 1, VEC><<<grid, 256, 0, s>>>(g); return true;
-    case 3 * 4 + 2: grouped_f32_kernel<3, 2, VEC><<<grid, 256, 0, s>>>(g); return true;
-    case 1 * 4 + 1: grouped_f32_kernel<1, 1, VEC><<<grid, 256, 0, s>>>(g); return true;
-    case 1 * 4 + 2: grouped_f32_kernel<1, 2, VEC><<<grid, 256, 0, s>>>(g); return true;
-    case 2 * 4 + 2: grouped_f32_kernel<2, 2, VEC><<<grid, 256, 0, s>>>(g); return true;
+  switch(big * 8 + small) {
+    case 3 * 8 + 3: launch_one<3, 3, VEC>(g, grid, s); return true;
+    case 3 * 8 + 1: launch_one<3, 1, VEC>(g, grid, s); return true;
+    case 3 * 8 + 2: launch_one<3, 2, VEC>(g, grid, s); return true;
+    case 1 * 8 + 1: launch_one<1, 1, VEC>(g, grid, s); return true;
+    case 1 * 8 + 2: launch_one<1, 2, VEC>(g, grid, s); return true;
+    case 2 * 8 + 2: launch_one<2, 2, VEC>(g, grid, s); return true;
+    case 4 * 8 + 4: launch_one<4, 4, VEC>(g, grid, s); return true;
+    case 4 * 8 + 5: launch_one<4, 5, VEC>(g, grid, s); return true;
+    case 4 * 8 + 6: launch_one<4, 6, VEC>(g, grid, s); return true;
+    case 5 * 8 + 5: launch_one<5, 5, VEC>(g, grid, s); return true;
+    case 5 * 8 + 6: launch_one<5, 6, VEC>(g, grid, s); return true;
+    case 6 * 8 + 6: launch_one<6, 6, VEC>(g, grid, s); return true;
   }
   return false;
 }

@@ -63,7 +63,8 @@ SIGNATURES = {
     "mfx_autotune_grouped": (_ST, [_V, _ST, ctypes.POINTER(Problem), _I32, ctypes.POINTER(Dep), _I32,
                                    ctypes.POINTER(_I32), _V]),
     "mfx_grouped_policy_count": (_I32, []),
-    "mfx_grouped_policy_info": (_ST, [_I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_D)]),
+    "mfx_grouped_policy_info": (_ST, [_I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_D),
+                                      ctypes.POINTER(_I32)]),
     "mfx_force_grouped_policy": (_ST, [_V, _I32]),
     "mfx_gemm_f64": (_ST, [_V, _ST, _ST, _I64, _I64, _I64, _D, _V, _I64, _V, _I64, _D, _V, _I64, _V]),
     "mfx_convert_f32_f64": (_ST, [_V, _I64, _I64, _V, _I64, _V, _I64, _V]),
@@ -325,8 +326,8 @@ def grouped_policy_count():
 
 
 def grouped_policy_info(policy):
-    """(large config, small config, waste bound) of a grouped tile policy."""
-    big, small, bound = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
-    _check(load().mfx_grouped_policy_info(policy, ctypes.byref(big), ctypes.byref(small), ctypes.byref(bound)),
-           "mfx_grouped_policy_info")
-    return big.value, small.value, bound.value
+    """(large config, small config, waste bound, one launch per level) of a grouped tile policy."""
+    big, small, bound, lv = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double(), ctypes.c_int32()
+    _check(load().mfx_grouped_policy_info(policy, ctypes.byref(big), ctypes.byref(small), ctypes.byref(bound),
+                                          ctypes.byref(lv)), "mfx_grouped_policy_info")
+    return big.value, small.value, bound.value, bool(lv.value)

@@ -58,7 +58,7 @@ def test_grouped_uneven_problems_vs_fp64(handle, mfx, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "split-bf16"])
-@pytest.mark.parametrize("config", [0, 1, 2, 3])
+@pytest.mark.parametrize("config", [0, 1, 2, 3, 4, 5, 6])
 def test_grouped_bitwise_equals_single_launch(handle, mfx, prec, config):
     rng = np.random.default_rng(2 + config)
     keep = rand_problems(mfx, rng, 12, hi=520)
@@ -213,7 +213,7 @@ def test_every_tile_policy(handle, mfx, prec):
     keep = rand_problems(mfx, rng, 24, lo=16, hi=700)
     try:
         for pol in range(mfx.grouped_policy_count()):
-            big, small, bound = mfx.grouped_policy_info(pol)
+            big, small, bound, per_level = mfx.grouped_policy_info(pol)
             fill(handle, keep, 6)
             C0 = [k[2].clone() for k in keep]
             probs = [mfx.problem(A, B, C, alpha=0.5, beta=2.0, transA=ta, transB=tb) for A, B, C, ta, tb, _ in keep]
@@ -263,3 +263,24 @@ def test_grouped_odd_shapes_scalar_loads(handle, mfx, prec):
     for A, B, C, ta, tb, c0 in keep:
         want = fp64(A, B, c0, 1.25, -0.5, ta, tb)
         assert rel(C, want) < TOL[prec], (C.shape, A.shape)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "split-bf16"])
+def test_chain_every_tile_policy(handle, mfx, prec):
+    """Dependent levels under every policy, both one persistent launch with
+    in-kernel waits and one launch per level."""
+    n, count = 320, 8
+    mats = [torch.empty((n, n), device=DEV) for _ in range(count)]
+    for j, x in enumerate(mats):
+        handle.fill_uniform(x, 13, j)
+    want = chain_fp64(mats)
+    try:
+        for pol in range(mfx.grouped_policy_count()):
+            scratch = [torch.empty((n, n), device=DEV) for _ in range(count - 2)]
+            out = torch.empty((n, n), device=DEV)
+            handle.force_grouped_policy(pol)
+            handle.chain(mats, scratch, out, prec=prec)
+            handle.grouped_status()
+            assert rel(out, want) < TOL[prec] * 3, pol
+    finally:
+        handle.force_grouped_policy(-1)
