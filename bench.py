@@ -65,7 +65,7 @@ def time_steps(torch, h, A, B, C, prec, steps, warmup, alpha, beta):
 def grouped_workload(torch, mfx, h, steps, warmup, seed=1):
     """512 independent problems (M, N, K log-uniform on [64, 2048], multiples of
     16, from `seed`) plus an 8-long chain of 2048 x 2048 matrices, as one
-    persistent grouped launch in a hipGraph with autotuned tiles, split-BF16."""
+    grouped launch (autotuned tile policy) captured in a hipGraph, split-BF16."""
     import numpy as np
     rng = np.random.default_rng(seed)
     dev = "cuda"
@@ -143,7 +143,7 @@ def grouped_workload(torch, mfx, h, steps, warmup, seed=1):
     t_single = s0.elapsed_time(s1)
     graph.close()
     return {"workload": "512 uneven problems (M,N,K log-uniform [64,2048], x16, seed 1) + 8-long chain of "
-                        "2048^2, split-BF16, one persistent grouped launch replayed from a hipGraph",
+                        "2048^2, split-BF16, one grouped launch (autotuned tile policy) replayed from a hipGraph",
             "flop": flop, "ms": round(ms, 4), "tflops": round(flop / (ms / 1e3) / 1e12, 2),
             "per_problem_launches_ms": round(t_single, 4),
             "per_problem_launches_tflops": round(flop / (t_single / 1e3) / 1e12, 2),
