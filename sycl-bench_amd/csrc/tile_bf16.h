@@ -219,27 +219,38 @@ __device__ __forceinline__ void bf16_tile(const BfTileArgs& t, int tm, int tn, u
   // and one i, the 64 lanes' chunks c = wave*64 + i*NT + lane are 64
   // consecutive chunks both in the packed operand (RUN is a multiple of 64)
   // and in the LDS stage, so each instruction moves one contiguous 1 KiB.
-  const int wave_base = tid - lane;
-  auto issue = [&](int s, int buf) __attribute__((always_inline)) {
-    const int prod = s / KT, kt = s - prod * KT;
-    const u32x4* A = prod ? t.PB : t.PA;
-    const u32x4* B = prod ? t.PA : t.PB;
-    const int last_a = (prod ? t.nblk_b : t.nblk_a) - 1, last_b = (prod ? t.nblk_a : t.nblk_b) - 1;
-    const int64_t koff = (int64_t)kt * RUN;
-    u32x4* sa = lds + buf * (SA + SB);
-    u32x4* sb = sa + SA;
+  // The LDS base of a wave is uniform: readfirstlane lets the compiler keep it
+  // (and so M0) in scalar registers.  Per-thread chunk offsets within the
+  // tile's A and B panels are computed once (32-bit: at most BM/32 blocks of
+  // one panel); a stage only moves the wave-uniform panel base along k.
+  const int wave_base = __builtin_amdgcn_readfirstlane(tid - lane);
+  int off_a[2][Cfg::PER_A], off_b[2][Cfg::PER_B];
+#pragma unroll
+  for(int p = 0; p < 2; ++p) {
+    const int last_a = (p ? t.nblk_b : t.nblk_a) - 1, last_b = (p ? t.nblk_a : t.nblk_b) - 1;
 #pragma unroll
     for(int i = 0; i < Cfg::PER_A; ++i) {
       const int c = tid + i * NT;
       const int blk = c / RUN, run = c - blk * RUN;
-      lds_dma16(A + (int64_t)min(tm * (BM / 32) + blk, last_a) * blk_stride + koff + run, sa + wave_base + i * NT);
+      off_a[p][i] = (min(tm * (BM / 32) + blk, last_a) - tm * (BM / 32)) * (int)blk_stride + run;
     }
 #pragma unroll
     for(int i = 0; i < Cfg::PER_B; ++i) {
       const int c = tid + i * NT;
       const int blk = c / RUN, run = c - blk * RUN;
-      lds_dma16(B + (int64_t)min(tn * (BN / 32) + blk, last_b) * blk_stride + koff + run, sb + wave_base + i * NT);
+      off_b[p][i] = (min(tn * (BN / 32) + blk, last_b) - tn * (BN / 32)) * (int)blk_stride + run;
     }
+  }
+  auto issue = [&](int s, int buf) __attribute__((always_inline)) {
+    const int prod = s / KT, kt = s - prod * KT;
+    const u32x4* A = (prod ? t.PB : t.PA) + ((int64_t)tm * (BM / 32) * blk_stride + (int64_t)kt * RUN);
+    const u32x4* B = (prod ? t.PA : t.PB) + ((int64_t)tn * (BN / 32) * blk_stride + (int64_t)kt * RUN);
+    u32x4* sa = lds + buf * (SA + SB);
+    u32x4* sb = sa + SA;
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_A; ++i) lds_dma16(A + (prod ? off_a[1][i] : off_a[0][i]), sa + wave_base + i * NT);
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_B; ++i) lds_dma16(B + (prod ? off_b[1][i] : off_b[0][i]), sb + wave_base + i * NT);
   };
   auto compute = [&](int cur) __attribute__((always_inline)) {
     const u32x4* sa = lds + cur * (SA + SB);
