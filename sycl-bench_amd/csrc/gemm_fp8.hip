@@ -38,6 +38,7 @@ struct QuantParams {
   int64_t ld;
   int rows, K;
   int kmajor;       // element (r, k) at src[r*ld + k], else src[k*ld + r]
+  int vec;          // kmajor with 16-byte aligned rows: 128-bit loads
   int kb128s;       // padded K / 128
   int two_term;
   u32x4* q1;        // packed, fragment order
@@ -61,8 +62,27 @@ __device__ __forceinline__ float block_max(float v, float* sh) {
   return m;
 }
 
+// e4m3 codes of four floats, packed little-endian into one word, by the
+// hardware conversion (v_cvt_pk_fp8_f32: OCP E4M3FN, round to nearest even).
+// Inputs are scaled so |x| <= 448 (the block scale guarantees it), so no
+// saturation is needed.
+__device__ __forceinline__ uint32_t e4m3_pack4(float a, float b, float c, float d) {
+  int w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+  w = __builtin_amdgcn_cvt_pk_fp8_f32(c, d, w, true);
+  return (uint32_t)w;
+}
+
+// Value of byte `j` (0..3) of a word of e4m3 codes (v_cvt_f32_fp8).
+template <int J>
+__device__ __forceinline__ float e4m3_byte(uint32_t w) {
+  return __builtin_amdgcn_cvt_f32_fp8((int)w, J);
+}
+
 // One workgroup per (row block, k block) of `block` x `block` elements;
-// thread t handles chunks of 16 consecutive k of one row.
+// thread t handles chunks of 16 consecutive k of one row.  k-contiguous
+// sources put consecutive threads along k (64-byte vector loads per
+// thread); row-contiguous ones put them along the rows (coalesced across
+// lanes).
 __global__ __launch_bounds__(256) void quantize_fp8_kernel(const QuantParams p) {
   __shared__ float sh[4];
   const int rb = blockIdx.y, kb = blockIdx.x;
@@ -74,28 +94,47 @@ __global__ __launch_bounds__(256) void quantize_fp8_kernel(const QuantParams p) 
 #pragma unroll
   for(int c = 0; c < 4; ++c) {
     const int id = threadIdx.x + c * 256;
-    const int r = rb * B + id / per_row, k0 = kb * B + (id % per_row) * 16;
+    const int rr = p.kmajor ? id / per_row : id % B, kk = p.kmajor ? id % per_row : id / B;
+    const int r = rb * B + rr, k0 = kb * B + kk * 16;
+    const bool live = id < chunks && r < p.rows;
+    if(p.kmajor) {
+      const float* src = p.src + (int64_t)r * p.ld + k0;
+      if(live && p.vec && k0 + 16 <= p.K) {
 #pragma unroll
-    for(int e = 0; e < 16; ++e) {
-      float x = 0.0f;
-      if(id < chunks && r < p.rows && k0 + e < p.K)
-        x = p.kmajor ? p.src[(int64_t)r * p.ld + k0 + e] : p.src[(int64_t)(k0 + e) * p.ld + r];
-      v[c][e] = x;
-      amax = fmaxf(amax, fabsf(x));
+        for(int q = 0; q < 4; ++q) {
+          const float4 x = reinterpret_cast<const float4*>(src)[q];
+          v[c][4 * q] = x.x;
+          v[c][4 * q + 1] = x.y;
+          v[c][4 * q + 2] = x.z;
+          v[c][4 * q + 3] = x.w;
+        }
+      } else {
+#pragma unroll
+        for(int e = 0; e < 16; ++e) v[c][e] = (live && k0 + e < p.K) ? src[e] : 0.0f;
+      }
+    } else {
+#pragma unroll
+      for(int e = 0; e < 16; ++e) v[c][e] = (live && k0 + e < p.K) ? p.src[(int64_t)(k0 + e) * p.ld + r] : 0.0f;
     }
+#pragma unroll
+    for(int e = 0; e < 16; ++e) amax = fmaxf(amax, fabsf(v[c][e]));
   }
   const int ex1 = fp8_block_exponent(block_max(amax, sh));
-  const float inv1 = ldexpf(1.0f, -ex1);
-  uint8_t q[4][16];
+  const float inv1 = ldexpf(1.0f, -ex1), s1 = ldexpf(1.0f, ex1);
+  uint32_t w1[4][4];
   float amax2 = 0.0f;
 #pragma unroll
   for(int c = 0; c < 4; ++c)
 #pragma unroll
-    for(int e = 0; e < 16; ++e) {
-      q[c][e] = e4m3_rne_sat(v[c][e] * inv1);
+    for(int j = 0; j < 4; ++j) {
+      float* x = &v[c][4 * j];
+      w1[c][j] = e4m3_pack4(x[0] * inv1, x[1] * inv1, x[2] * inv1, x[3] * inv1);
       if(p.two_term) {
-        v[c][e] -= ldexpf(e4m3_to_f32(q[c][e]), ex1);
-        amax2 = fmaxf(amax2, fabsf(v[c][e]));
+        x[0] -= e4m3_byte<0>(w1[c][j]) * s1;
+        x[1] -= e4m3_byte<1>(w1[c][j]) * s1;
+        x[2] -= e4m3_byte<2>(w1[c][j]) * s1;
+        x[3] -= e4m3_byte<3>(w1[c][j]) * s1;
+        amax2 = fmaxf(amax2, fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fmaxf(fabsf(x[2]), fabsf(x[3]))));
       }
     }
   int ex2 = 0;
@@ -112,40 +151,24 @@ __global__ __launch_bounds__(256) void quantize_fp8_kernel(const QuantParams p) 
   for(int c = 0; c < 4; ++c) {
     const int id = threadIdx.x + c * 256;
     if(id >= chunks) continue;
-    const int r = rb * B + id / per_row, k0 = kb * B + (id % per_row) * 16;
-    uint32_t w1[4], w2[4];
+    const int rr = p.kmajor ? id / per_row : id % B, kk = p.kmajor ? id % per_row : id / B;
+    const int r = rb * B + rr, k0 = kb * B + kk * 16;
+    uint32_t w2[4] = {0, 0, 0, 0};
+    if(p.two_term)
 #pragma unroll
-    for(int j = 0; j < 4; ++j) {
-      w1[j] = (uint32_t)q[c][4 * j] | ((uint32_t)q[c][4 * j + 1] << 8) | ((uint32_t)q[c][4 * j + 2] << 16) |
-              ((uint32_t)q[c][4 * j + 3] << 24);
-      if(p.two_term) {
-        uint8_t b[4];
-        for(int t = 0; t < 4; ++t) b[t] = e4m3_rne_sat(v[c][4 * j + t] * inv2);
-        w2[j] = (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24);
-      }
-    }
+      for(int j = 0; j < 4; ++j)
+        w2[j] = e4m3_pack4(v[c][4 * j] * inv2, v[c][4 * j + 1] * inv2, v[c][4 * j + 2] * inv2, v[c][4 * j + 3] * inv2);
     if(p.q1) {
       // fragment-order position of (r, k0..k0+15)
       const int kb64s = p.kb128s * 2;
       const int blk = r / 32, kb64 = k0 / 64, w64 = k0 % 64;
       const int lane = (r % 32) + 32 * (w64 / 32), half = (w64 % 32) / 16;
       const int64_t at = (((int64_t)blk * kb64s + kb64) * 2 + half) * 64 + lane;
-      u32x4 c1;
-      c1.x = w1[0];
-      c1.y = w1[1];
-      c1.z = w1[2];
-      c1.w = w1[3];
-      p.q1[at] = c1;
-      if(p.two_term) {
-        u32x4 c2;
-        c2.x = w2[0];
-        c2.y = w2[1];
-        c2.z = w2[2];
-        c2.w = w2[3];
-        p.q2[at] = c2;
-      }
+      p.q1[at] = (u32x4){w1[c][0], w1[c][1], w1[c][2], w1[c][3]};
+      if(p.two_term) p.q2[at] = (u32x4){w2[0], w2[1], w2[2], w2[3]};
     } else if(r < p.rows) {
-      for(int e = 0; e < 16 && k0 + e < p.K; ++e) p.plain[(int64_t)r * p.plain_ld + k0 + e] = q[c][e];
+      for(int e = 0; e < 16 && k0 + e < p.K; ++e)
+        p.plain[(int64_t)r * p.plain_ld + k0 + e] = (uint8_t)(w1[c][e / 4] >> (8 * (e % 4)));
     }
   }
 }
@@ -345,6 +368,7 @@ mfx_status quantize(Handle* h, const float* src, int64_t ld, int rows, int K, bo
   p.rows = rows;
   p.K = K;
   p.kmajor = kmajor;
+  p.vec = kmajor && (reinterpret_cast<uintptr_t>(src) & 15u) == 0 && ld % 4 == 0;
   p.kb128s = (int)kb128s;
   p.two_term = two;
   p.q1 = q1;
@@ -443,6 +467,7 @@ mfx_status launch_quantize_fp8(Handle* h, int64_t rows, int64_t cols, const floa
   p.rows = (int)rows;
   p.K = (int)cols;
   p.kmajor = 1;
+  p.vec = (reinterpret_cast<uintptr_t>(x) & 15u) == 0 && ldx % 4 == 0;
   p.kb128s = (int)ceil_div(cols, block);
   p.two_term = 0;
   p.e1 = exps;
