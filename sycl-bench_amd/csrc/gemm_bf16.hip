@@ -91,7 +91,7 @@ PackPlan plan(const DenseOp& op, bool split, int config) {
   pl.rows_pad_a = round_up(op.M, 32);
   pl.rows_pad_b = round_up(op.N, 32);
   (void)c;
-  pl.kbs = round_up(op.K, 16 * KB_PER_STAGE) / 16;
+  pl.kbs = round_up(op.K, 16 * kb_per_stage_of(nhl)) / 16;
   pl.bytes_a = (size_t)pl.rows_pad_a * pl.kbs * 16 * nhl * 2;
   pl.bytes_b = (size_t)pl.rows_pad_b * pl.kbs * 16 * nhl * 2;
   return pl;
@@ -126,11 +126,11 @@ PackSrc make_pack_src(const float* src, int64_t ld, int rows, int K, bool kmajor
 }
 
 int bf_config_count() { return kBfConfigCount; }
-bool bf_config_info(int c, int* bm, int* bn, int* bk, int* waves) {
+bool bf_config_info(int c, bool split, int* bm, int* bn, int* bk, int* waves) {
   if(c < 0 || c >= kBfConfigCount) return false;
   *bm = kBfConfigs[c].bm;
   *bn = kBfConfigs[c].bn;
-  *bk = 16 * KB_PER_STAGE;
+  *bk = 16 * kb_per_stage_of(split ? 2 : 1);
   *waves = kBfConfigs[c].waves;
   return true;
 }

@@ -313,7 +313,7 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
       // load_chunk8 takes 128-bit loads along k only
       P.vec_a = P.a_kmajor && P.vec_a;
       P.vec_b = P.b_kmajor && P.vec_b;
-      P.kbs = (int)(round_up(q.k, 16 * KB_PER_STAGE) / 16);
+      P.kbs = (int)(round_up(q.k, 16 * kb_per_stage_of(nhl)) / 16);
       P.pack_slices = (int)ceil_div(P.kbs, PACK_KBLOCKS);
       packed_bytes += packed_sizes(P, nhl).first + packed_sizes(P, nhl).second;
       if(P.ndeps == 0 || pl.per_level) {

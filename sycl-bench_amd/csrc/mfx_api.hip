@@ -12,7 +12,7 @@ namespace mfx {
 int f32_config_count();
 bool f32_config_info(int c, int* bm, int* bn, int* bk, int* waves);
 int bf_config_count();
-bool bf_config_info(int c, int* bm, int* bn, int* bk, int* waves);
+bool bf_config_info(int c, bool split, int* bm, int* bn, int* bk, int* waves);
 int fp8_config_count();
 bool fp8_config_info(int c, int* bm, int* bn, int* bk, int* waves);
 mfx_status launch_gemm_f64(Handle* h, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
@@ -101,8 +101,8 @@ int config_count(mfx_precision p) {
 bool config_info(mfx_precision p, int c, int* bm, int* bn, int* bk, int* waves) {
   switch(p) {
     case MFX_PREC_FP32: return f32_config_info(c, bm, bn, bk, waves);
-    case MFX_PREC_SPLIT_BF16:
-    case MFX_PREC_BF16: return bf_config_info(c, bm, bn, bk, waves);
+    case MFX_PREC_SPLIT_BF16: return bf_config_info(c, true, bm, bn, bk, waves);
+    case MFX_PREC_BF16: return bf_config_info(c, false, bm, bn, bk, waves);
     case MFX_PREC_FP8_BLOCK:
     case MFX_PREC_FP8_BLOCK2: return fp8_config_info(c, bm, bn, bk, waves);
   }

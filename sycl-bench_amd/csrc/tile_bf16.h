@@ -24,7 +24,14 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 // promoted to registers instead of living in scratch memory.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int KB_PER_STAGE = 2;  // BK = 32
+// 16-wide k blocks per LDS stage: 2 (BK = 32) for split BF16, whose hi/lo
+// planes fill a 64 KiB stage of a 256x256 tile; 4 (BK = 64) for single BF16,
+// so its stages carry as many bytes and MFMAs per barrier.
+template <int NHL>
+constexpr int kb_per_stage() {
+  return NHL == 2 ? 2 : 4;
+}
+__host__ __device__ inline int kb_per_stage_of(int nhl) { return nhl == 2 ? 2 : 4; }
 
 __device__ __forceinline__ uint32_t pack_bf16_pair(uint16_t a, uint16_t b) {
   return (uint32_t)a | ((uint32_t)b << 16);
@@ -116,8 +123,9 @@ template <int BM, int BN, int WM, int WN, int NHL>
 struct BfCfg {
   static constexpr int NWM = BM / WM, NWN = BN / WN, NW = NWM * NWN, NT = NW * 64;
   static constexpr int FM = WM / 32, FN = WN / 32;
-  static constexpr int CH_A = BM / 32 * KB_PER_STAGE * NHL * 64;  // 16-byte chunks per stage
-  static constexpr int CH_B = BN / 32 * KB_PER_STAGE * NHL * 64;
+  static constexpr int KB = kb_per_stage<NHL>();
+  static constexpr int CH_A = BM / 32 * KB * NHL * 64;  // 16-byte chunks per stage
+  static constexpr int CH_B = BN / 32 * KB * NHL * 64;
   static constexpr int PER_A = CH_A / NT, PER_B = CH_B / NT;
   static constexpr int SMEM_CHUNKS = 2 * (CH_A + CH_B);
   static_assert(CH_A % NT == 0 && CH_B % NT == 0, "stage must split evenly");
@@ -145,14 +153,14 @@ __device__ __forceinline__ void bf16_kb_mma(floatx16 (&acc)[FM][FN], const u32x4
 #pragma unroll
   for(int i = 0; i < FM; ++i) {
     const int blk = wm * FM + i;
-    ah[i] = as_bf16x8(sa[(blk * KB_PER_STAGE + kb) * NHL * 64 + lane]);
-    if(NHL == 2) al[i] = as_bf16x8(sa[((blk * KB_PER_STAGE + kb) * NHL + 1) * 64 + lane]);
+    ah[i] = as_bf16x8(sa[(blk * kb_per_stage<NHL>() + kb) * NHL * 64 + lane]);
+    if(NHL == 2) al[i] = as_bf16x8(sa[((blk * kb_per_stage<NHL>() + kb) * NHL + 1) * 64 + lane]);
   }
 #pragma unroll
   for(int j = 0; j < FN; ++j) {
     const int blk = wn * FN + j;
-    bh[j] = as_bf16x8(sb[(blk * KB_PER_STAGE + kb) * NHL * 64 + lane]);
-    if(NHL == 2) bl[j] = as_bf16x8(sb[((blk * KB_PER_STAGE + kb) * NHL + 1) * 64 + lane]);
+    bh[j] = as_bf16x8(sb[(blk * kb_per_stage<NHL>() + kb) * NHL * 64 + lane]);
+    if(NHL == 2) bl[j] = as_bf16x8(sb[((blk * kb_per_stage<NHL>() + kb) * NHL + 1) * 64 + lane]);
   }
   // Product-major order: consecutive MFMAs write different accumulators, so
   // none waits on the previous one's result.  The small cross terms go in
@@ -173,12 +181,12 @@ __device__ __forceinline__ void bf16_kb_mma(floatx16 (&acc)[FM][FN], const u32x4
     for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
 }
 
-// MFMAs of a whole LDS stage (KB_PER_STAGE k blocks).
+// MFMAs of a whole LDS stage (kb_per_stage k blocks).
 template <int FM, int FN, int NHL>
 __device__ __forceinline__ void bf16_stage_mma(floatx16 (&acc)[FM][FN], const u32x4* sa, const u32x4* sb, int wm,
     int wn, int lane) {
 #pragma unroll
-  for(int kb = 0; kb < KB_PER_STAGE; ++kb) bf16_kb_mma<FM, FN, NHL>(acc, sa, sb, wm, wn, lane, kb);
+  for(int kb = 0; kb < kb_per_stage<NHL>(); ++kb) bf16_kb_mma<FM, FN, NHL>(acc, sa, sb, wm, wn, lane, kb);
 }
 
 template <int FM, int FN>
@@ -207,8 +215,8 @@ __device__ __forceinline__ void bf16_tile(const BfTileArgs& t, int tm, int tn, u
 
   // Chunk c of a stage: block c / RUN of the tile, then the contiguous
   // (kb, hl, lane) run of that block for this stage.
-  constexpr int RUN = KB_PER_STAGE * NHL * 64;
-  const int KT = t.kbs / KB_PER_STAGE;
+  constexpr int RUN = Cfg::KB * NHL * 64;
+  const int KT = t.kbs / Cfg::KB;
   const int steps = KT * t.nprod;
   const int64_t blk_stride = (int64_t)t.kbs * NHL * 64;
   // Packed operands hold only the 32-row blocks that meet the matrix: a tile
