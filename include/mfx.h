@@ -104,6 +104,34 @@ mfx_status mfx_syr2k(mfx_handle handle, mfx_precision prec, mfx_uplo uplo, int m
     int64_t n, int64_t k, float alpha, const float* A, int64_t lda, const float* B,
     int64_t ldb, float beta, float* C, int64_t ldc, mfx_stream stream);
 
+/* ---- covariance / correlation --------------------------------------------- */
+
+/*
+ * Covariance of the n x m row-major `data` (n observations of m variables):
+ * mean[j] = (sum_i data[i][j]) / n_const (float division), centered = data -
+ * mean (n x m, ldw), symmat = centeredᵀ · centered (m x m, full square: upper
+ * triangle computed, mirrored).  `prec` selects the contraction (FP32,
+ * SPLIT_BF16, BF16).  `data` is not modified, so repeated calls agree.
+ * Replaces: polybench/covariance.cpp:76-128 (kernels CovarianceMean,
+ * CovarianceReduce, CovarianceCovar; the reference centres `data` in place).
+ */
+mfx_status mfx_covariance(mfx_handle handle, mfx_precision prec, int64_t m, int64_t n,
+    const float* data, int64_t ld, float n_const, float* mean, float* centered, int64_t ldw,
+    float* symmat, int64_t lds, mfx_stream stream);
+/*
+ * Correlation of the n x m `data`: mean as above with (float)n_const;
+ * stddev[j] = sqrt((sum_i (data[i][j] - mean[j])^2) / n_const), replaced by 1
+ * when <= eps; centered = (data - mean) / sqrt(n_const) / stddev; symmat =
+ * centeredᵀ · centered (full square) with a unit diagonal.
+ * Replaces: polybench/correlation.cpp:92-158 (kernels CorrelationMean,
+ * CorrelationStd, CorrelationReduce, CorrelationCorr, Correlation5; the
+ * reference accumulates mean / stddev onto their previous contents and
+ * centres `data` in place, so its repeated runs drift; these calls do not).
+ */
+mfx_status mfx_correlation(mfx_handle handle, mfx_precision prec, int64_t m, int64_t n,
+    const float* data, int64_t ld, double n_const, double eps, float* mean, float* stddev,
+    float* centered, int64_t ldw, float* symmat, int64_t lds, mfx_stream stream);
+
 /* ---- grouped / chained launches ------------------------------------------ */
 
 typedef struct mfx_gemm_problem {

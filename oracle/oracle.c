@@ -169,6 +169,82 @@ void syr2k_cpu(const DATA_TYPE* A, const DATA_TYPE* B, DATA_TYPE* C, size_t size
       }
 }
 
+/* polybench/covariance.cpp:18 */
+static const DATA_TYPE COV_FLOAT_N = 3214212.01f;
+
+/* polybench/covariance.cpp:20-29 -- into a zeroed (size+1)^2 array, stride
+ * size+1: rows and columns 0..size-1 set, row and column `size` left zero. */
+void covariance_init(DATA_TYPE* data, size_t size) {
+  const size_t M = size, N = size;
+  for(size_t i = 0; i < M; i++)
+    for(size_t j = 0; j < N; j++) data[i * (N + 1) + j] = ((DATA_TYPE)i * j) / M;
+}
+
+/* polybench/covariance.cpp:31-61 -- 1-based; `data` is centred in place. */
+void covariance_cpu(DATA_TYPE* data, DATA_TYPE* symmat, DATA_TYPE* mean, size_t size) {
+  const size_t M = size, N = size;
+  for(size_t j = 1; j <= M; j++) {
+    mean[j] = 0.0f;
+    for(size_t i = 1; i <= N; i++) mean[j] += data[i * (M + 1) + j];
+    mean[j] /= COV_FLOAT_N;
+  }
+  for(size_t i = 1; i <= N; i++)
+    for(size_t j = 1; j <= M; j++) data[i * (M + 1) + j] -= mean[j];
+  for(size_t j1 = 1; j1 <= M; j1++)
+    for(size_t j2 = j1; j2 <= M; j2++) {
+      symmat[j1 * (M + 1) + j2] = 0.0f;
+      for(size_t i = 1; i <= N; i++) symmat[j1 * (M + 1) + j2] += data[i * (M + 1) + j1] * data[i * (M + 1) + j2];
+      symmat[j2 * (M + 1) + j1] = symmat[j1 * (M + 1) + j2];
+    }
+}
+
+/* polybench/correlation.cpp:8-9 */
+static const double CORR_FLOAT_N = 3214212.01;
+static const double CORR_EPS = 0.005;
+
+/* polybench/correlation.cpp:17-25 -- into a zeroed (size+1)^2 array with row
+ * stride `size` (not size+1), i and j from 0 to size inclusive; later writes
+ * win where flat indices coincide. */
+void correlation_init(DATA_TYPE* data, size_t size) {
+  const size_t M = size, N = size;
+  for(size_t i = 0; i <= M; i++)
+    for(size_t j = 0; j <= N; j++) data[i * N + j] = ((DATA_TYPE)i * j) / (M + 1);
+}
+
+/* polybench/correlation.cpp:27-70 -- 1-based; float statistics divided by the
+ * double constant where the reference divides a float by a double. */
+void correlation_cpu(DATA_TYPE* data, DATA_TYPE* mean, DATA_TYPE* stddev, DATA_TYPE* symmat, size_t size) {
+  const size_t M = size, N = size;
+  for(size_t j = 1; j <= M; j++) {
+    mean[j] = 0.0f;
+    for(size_t i = 1; i <= N; i++) mean[j] += data[i * (M + 1) + j];
+    mean[j] /= (DATA_TYPE)CORR_FLOAT_N;
+  }
+  for(size_t j = 1; j <= M; j++) {
+    stddev[j] = 0.0f;
+    for(size_t i = 1; i <= N; i++)
+      stddev[j] += (data[i * (M + 1) + j] - mean[j]) * (data[i * (M + 1) + j] - mean[j]);
+    stddev[j] = (DATA_TYPE)((double)stddev[j] / CORR_FLOAT_N);
+    stddev[j] = (DATA_TYPE)sqrt((double)stddev[j]);
+    stddev[j] = stddev[j] <= CORR_EPS ? 1.0f : stddev[j];
+  }
+  for(size_t i = 1; i <= N; i++)
+    for(size_t j = 1; j <= M; j++) {
+      data[i * (M + 1) + j] -= mean[j];
+      data[i * (M + 1) + j] = (DATA_TYPE)((double)data[i * (M + 1) + j] / sqrt(CORR_FLOAT_N));
+      data[i * (M + 1) + j] /= stddev[j];
+    }
+  for(size_t j1 = 1; j1 <= M - 1; j1++) {
+    symmat[j1 * (M + 1) + j1] = 1.0f;
+    for(size_t j2 = j1 + 1; j2 <= M; j2++) {
+      symmat[j1 * (M + 1) + j2] = 0.0f;
+      for(size_t i = 1; i <= N; i++) symmat[j1 * (M + 1) + j2] += data[i * (M + 1) + j1] * data[i * (M + 1) + j2];
+      symmat[j2 * (M + 1) + j1] = symmat[j1 * (M + 1) + j2];
+    }
+  }
+  symmat[M * (M + 1) + M] = 1.0f;
+}
+
 /* runtime/matmulchain.cpp:62-70 -- identity inputs. */
 void matmulchain_identity(DATA_TYPE* M, size_t n) {
   for(size_t i = 0; i < n; ++i)

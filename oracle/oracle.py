@@ -46,6 +46,10 @@ _sig("mm2_cpu", None, _F32P, _F32P, _F32P, _F32P, _F32P, _SZ)
 _sig("mm3_init", None, _F32P, _F32P, _F32P, _F32P, _SZ)
 _sig("mm3_cpu", None, _F32P, _F32P, _F32P, _F32P, _F32P, _F32P, _F32P, _SZ)
 _sig("syrk_init", None, _F32P, _F32P, _SZ)
+_sig("covariance_init", None, _F32P, _SZ)
+_sig("covariance_cpu", None, _F32P, _F32P, _F32P, _SZ)
+_sig("correlation_init", None, _F32P, _SZ)
+_sig("correlation_cpu", None, _F32P, _F32P, _F32P, _F32P, _SZ)
 _sig("syrk_cpu", None, _F32P, _F32P, _SZ)
 _sig("syr2k_init", None, _F32P, _F32P, _F32P, _SZ)
 _sig("syr2k_cpu", None, _F32P, _F32P, _F32P, _SZ)
@@ -145,6 +149,43 @@ def syr2k_inputs(n):
     A, B, C = _z(n * n), _z(n * n), _z(n * n)
     _lib.syr2k_init(A, B, C, n)
     return A.reshape(n, n), B.reshape(n, n), C.reshape(n, n)
+
+
+COV_FLOAT_N = np.float32(3214212.01)  # polybench/covariance.cpp:18
+CORR_FLOAT_N = 3214212.01              # polybench/correlation.cpp:8
+CORR_EPS = 0.005                       # polybench/correlation.cpp:9
+
+
+def covariance_inputs(n):
+    """(n+1) x (n+1) data of polybench/covariance.cpp:20-29 (used 1-based)."""
+    data = _z((n + 1) * (n + 1))
+    _lib.covariance_init(data, n)
+    return data.reshape(n + 1, n + 1)
+
+
+def covariance_reference(n):
+    """(data_in, symmat) of polybench/covariance.cpp:20-61, (n+1) x (n+1), 1-based."""
+    data = covariance_inputs(n).copy().reshape(-1)
+    d_in = data.copy().reshape(n + 1, n + 1)
+    symmat, mean = _z((n + 1) * (n + 1)), _z(n + 1)
+    _lib.covariance_cpu(data, symmat, mean, n)
+    return d_in, symmat.reshape(n + 1, n + 1)
+
+
+def correlation_inputs(n):
+    """(n+1) x (n+1) data of polybench/correlation.cpp:17-25 (row stride n in the flat array)."""
+    data = _z((n + 1) * (n + 1))
+    _lib.correlation_init(data, n)
+    return data.reshape(n + 1, n + 1)
+
+
+def correlation_reference(n):
+    """(data_in, symmat) of polybench/correlation.cpp:17-70, (n+1) x (n+1), 1-based."""
+    data = correlation_inputs(n).copy().reshape(-1)
+    d_in = data.copy().reshape(n + 1, n + 1)
+    mean, stddev, symmat = _z(n + 1), _z(n + 1), _z((n + 1) * (n + 1))
+    _lib.correlation_cpu(data, mean, stddev, symmat, n)
+    return d_in, symmat.reshape(n + 1, n + 1)
 
 
 def matmulchain_identity(n):

@@ -214,6 +214,36 @@ mfx_status check_matrix(const char* name, const void* p, int64_t rows, int64_t c
   return MFX_SUCCESS;
 }
 
+// C (n x n) = alpha * A·Aᵀ (syr2k: A·Bᵀ + B·Aᵀ) + beta * C on the
+// triangle schedule.  A is n x k row-major, or with `trans` k x n (so the
+// product is Aᵀ·A, the covariance form).
+mfx_status sym_common(Handle* h, mfx_precision prec, mfx_uplo uplo, int mirror, int64_t n, int64_t k, float alpha,
+    const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int nprod, bool trans,
+    hipStream_t s) {
+  if(!valid_prec(prec)) return invalid("unknown precision");
+  if(uplo != MFX_LOWER && uplo != MFX_UPPER && uplo != MFX_FULL) return invalid("uplo must be LOWER, UPPER or FULL");
+  if(n < 0 || k < 0) return invalid("negative dimension");
+  MFX_CHECK(trans ? check_matrix("A", A, k, n, lda) : check_matrix("A", A, n, k, lda));
+  if(nprod == 2) MFX_CHECK(trans ? check_matrix("B", B, k, n, ldb) : check_matrix("B", B, n, k, ldb));
+  MFX_CHECK(check_matrix("C", C, n, n, ldc));
+  if(n == 0) return MFX_SUCCESS;
+  DenseOp op{};
+  op.M = op.N = (int)n;
+  op.K = (int)k;
+  op.A = A;
+  op.lda = lda;
+  op.a_kmajor = !trans;
+  op.B = nprod == 2 ? B : A;
+  op.ldb = nprod == 2 ? ldb : lda;
+  op.b_kmajor = !trans;
+  op.nprod = nprod;
+  const int mode = uplo == MFX_LOWER ? TILES_LOWER : uplo == MFX_UPPER ? TILES_UPPER : TILES_FULL;
+  op.tiles = mode;
+  op.ep = Epilogue{C, ldc, alpha, beta, (int)n, (int)n, mode, mode == TILES_FULL ? 0 : (mirror ? 1 : 0)};
+  const int cfg = mode == TILES_FULL ? choose_config(h, prec, n, n, k * nprod) : choose_square_config(h, prec, n, k * nprod);
+  return dispatch_dense(h, prec, op, cfg, s);
+}
+
 }  // namespace mfx
 
 using namespace mfx;
@@ -358,37 +388,11 @@ mfx_status mfx_gemm(mfx_handle handle, mfx_precision prec, mfx_op transA, mfx_op
   return dispatch_dense(h, prec, op, cfg, static_cast<hipStream_t>(stream));
 }
 
-static mfx_status sym_common(Handle* h, mfx_precision prec, mfx_uplo uplo, int mirror, int64_t n, int64_t k,
-    float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc,
-    int nprod, hipStream_t s) {
-  if(!valid_prec(prec)) return invalid("unknown precision");
-  if(uplo != MFX_LOWER && uplo != MFX_UPPER && uplo != MFX_FULL) return invalid("uplo must be LOWER, UPPER or FULL");
-  if(n < 0 || k < 0) return invalid("negative dimension");
-  MFX_CHECK(check_matrix("A", A, n, k, lda));
-  if(nprod == 2) MFX_CHECK(check_matrix("B", B, n, k, ldb));
-  MFX_CHECK(check_matrix("C", C, n, n, ldc));
-  if(n == 0) return MFX_SUCCESS;
-  DenseOp op{};
-  op.M = op.N = (int)n;
-  op.K = (int)k;
-  op.A = A;
-  op.lda = lda;
-  op.a_kmajor = true;
-  op.B = nprod == 2 ? B : A;
-  op.ldb = nprod == 2 ? ldb : lda;
-  op.b_kmajor = true;
-  op.nprod = nprod;
-  const int mode = uplo == MFX_LOWER ? TILES_LOWER : uplo == MFX_UPPER ? TILES_UPPER : TILES_FULL;
-  op.tiles = mode;
-  op.ep = Epilogue{C, ldc, alpha, beta, (int)n, (int)n, mode, mode == TILES_FULL ? 0 : (mirror ? 1 : 0)};
-  const int cfg = mode == TILES_FULL ? choose_config(h, prec, n, n, k * nprod) : choose_square_config(h, prec, n, k * nprod);
-  return dispatch_dense(h, prec, op, cfg, s);
-}
 
 mfx_status mfx_syrk(mfx_handle handle, mfx_precision prec, mfx_uplo uplo, int mirror, int64_t n, int64_t k, float alpha,
     const float* A, int64_t lda, float beta, float* C, int64_t ldc, mfx_stream stream) {
   HANDLE_OR_FAIL(handle);
-  return sym_common(h, prec, uplo, mirror, n, k, alpha, A, lda, nullptr, 0, beta, C, ldc, 1,
+  return sym_common(h, prec, uplo, mirror, n, k, alpha, A, lda, nullptr, 0, beta, C, ldc, 1, false,
       static_cast<hipStream_t>(stream));
 }
 
@@ -396,7 +400,7 @@ mfx_status mfx_syr2k(mfx_handle handle, mfx_precision prec, mfx_uplo uplo, int m
     float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc,
     mfx_stream stream) {
   HANDLE_OR_FAIL(handle);
-  return sym_common(h, prec, uplo, mirror, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 2,
+  return sym_common(h, prec, uplo, mirror, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 2, false,
       static_cast<hipStream_t>(stream));
 }
 

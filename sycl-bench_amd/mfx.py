@@ -66,6 +66,8 @@ SIGNATURES = {
     "mfx_grouped_policy_info": (_ST, [_I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_D),
                                       ctypes.POINTER(_I32)]),
     "mfx_force_grouped_policy": (_ST, [_V, _I32]),
+    "mfx_covariance": (_ST, [_V, _ST, _I64, _I64, _V, _I64, _F, _V, _V, _I64, _V, _I64, _V]),
+    "mfx_correlation": (_ST, [_V, _ST, _I64, _I64, _V, _I64, _D, _D, _V, _V, _V, _I64, _V, _I64, _V]),
     "mfx_gemm_f64": (_ST, [_V, _ST, _ST, _I64, _I64, _I64, _D, _V, _I64, _V, _I64, _D, _V, _I64, _V]),
     "mfx_convert_f32_f64": (_ST, [_V, _I64, _I64, _V, _I64, _V, _I64, _V]),
     "mfx_rel_frobenius": (_ST, [_V, _ST, _I64, _I64, _V, _I64, _V, _I64, ctypes.POINTER(_D), _V]),
@@ -173,6 +175,19 @@ class Handle:
         n, k = A.shape
         _check(self.lib.mfx_syr2k(self.h, prec_id(prec), uplo, int(mirror), n, k, alpha, _ptr(A), _ld(A), _ptr(B),
                                   _ld(B), beta, _ptr(C), _ld(C), _stream(stream)), "mfx_syr2k")
+
+    def covariance(self, data, mean, centered, symmat, n_const, prec="fp32", stream=None):
+        """data: n x m observations; mean (m), centered (n x m), symmat (m x m) are outputs."""
+        n, m = data.shape
+        _check(self.lib.mfx_covariance(self.h, prec_id(prec), m, n, _ptr(data), _ld(data), n_const, _ptr(mean),
+                                       _ptr(centered), _ld(centered), _ptr(symmat), _ld(symmat), _stream(stream)),
+               "mfx_covariance")
+
+    def correlation(self, data, mean, stddev, centered, symmat, n_const, eps, prec="fp32", stream=None):
+        n, m = data.shape
+        _check(self.lib.mfx_correlation(self.h, prec_id(prec), m, n, _ptr(data), _ld(data), n_const, eps, _ptr(mean),
+                                        _ptr(stddev), _ptr(centered), _ld(centered), _ptr(symmat), _ld(symmat),
+                                        _stream(stream)), "mfx_correlation")
 
     def gemm_f64(self, A, B, C, alpha=1.0, beta=0.0, transA=False, transB=False, stream=None):
         m, n = C.shape

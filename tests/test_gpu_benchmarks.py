@@ -35,6 +35,8 @@ REFERENCE_RUNS = [
     ("syrk", "Polybench_Syrk", 512),
     ("syr2k", "Polybench_Syr2k", 512),
     ("matmulchain", "MatmulChain", 512),
+    ("covariance", "Polybench_Covariance", 512),
+    ("correlation", "Polybench_Correlation", 512),
 ]
 
 
@@ -90,6 +92,16 @@ def test_matmulchain_grouped_launch(init):
     assert r.returncode == 0, r.stderr
     assert res["Verification"] == "PASS"
     assert res["_names"][0].endswith("_len8_grouped")
+
+
+@pytest.mark.parametrize("exe,n", [("covariance", 300), ("correlation", 257), ("covariance", 1024),
+                                   ("correlation", 1024)])
+@pytest.mark.parametrize("prec,init", [("fp32", "polybench"), ("split-bf16", "polybench"), ("bf16", "uniform"),
+                                       ("split-bf16", "uniform")])
+def test_covariance_correlation_programs(exe, n, prec, init):
+    r, res = bench(exe, f"--size={n}", "--num-runs=2", f"--precision={prec}", f"--init={init}", "--metrics=extended")
+    assert r.returncode == 0, r.stderr
+    assert res["Verification"] == "PASS", res
 
 
 @pytest.mark.parametrize("exe", ["2mm", "3mm"])

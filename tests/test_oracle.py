@@ -276,3 +276,72 @@ def test_cpu_baseline_computes_reference_gemm(oracle):
     got2 = C2.copy()
     oracle.cpu_baseline().cpu_syrk_rows(A2.ravel(), got2.ravel(), n, 123.0, 14512.0, 0, n)
     assert oracle.percent_diff_count(want2, got2) == 0
+
+
+# ---- covariance / correlation (polybench/covariance.cpp, polybench/correlation.cpp) ----
+
+def _stats_f64(x, n_const, correlation, eps=0.005):
+    """FP64 statistics of the 1-based interior x (n x m): covariance, or
+    correlation with a unit diagonal."""
+    mean = x.sum(axis=0) / n_const
+    c = x - mean
+    if correlation:
+        sd = np.sqrt((c * c).sum(axis=0) / n_const)
+        sd = np.where(sd <= eps, 1.0, sd)
+        c = c / np.sqrt(n_const) / sd
+    s = c.T @ c
+    if correlation:
+        np.fill_diagonal(s, 1.0)
+    return s
+
+
+def test_covariance_inputs_follow_the_reference_layout(oracle):
+    n = 7
+    d = oracle.covariance_inputs(n)
+    for i in range(n + 1):
+        for j in range(n + 1):
+            want = np.float32(np.float32(i * j) / n) if i < n and j < n else 0.0
+            assert d[i, j] == want
+
+
+def test_correlation_inputs_use_row_stride_n(oracle):
+    n = 6
+    flat = oracle.correlation_inputs(n).reshape(-1)
+    want = np.zeros((n + 1) * (n + 1), np.float32)
+    for i in range(n + 1):
+        for j in range(n + 1):
+            want[i * n + j] = np.float32(np.float32(i * j) / (n + 1))
+    np.testing.assert_array_equal(flat, want)
+
+
+@pytest.mark.parametrize("n", [16, 33, 96])
+def test_covariance_oracle_matches_fp64(oracle, n):
+    d, s = oracle.covariance_reference(n)
+    want = _stats_f64(d[1:, 1:].astype(np.float64), float(oracle.COV_FLOAT_N), False)
+    got = s[1:, 1:]
+    assert np.array_equal(got, got.T)
+    assert oracle.percent_diff_count(want, got) == 0
+
+
+@pytest.mark.parametrize("n", [16, 33, 300])
+def test_correlation_oracle_matches_fp64(oracle, n):
+    d, s = oracle.correlation_reference(n)
+    want = _stats_f64(d[1:, 1:].astype(np.float64), oracle.CORR_FLOAT_N, True)
+    got = s[1:, 1:]
+    assert np.array_equal(got, got.T)
+    assert np.all(np.diag(got) == 1.0)
+    assert oracle.percent_diff_count(want, got) == 0
+    # at n = 300 the deviations exceed eps and the correlations are O(1): the
+    # check above is then not vacuous (percentDiff skips values below 0.01)
+    if n == 300:
+        assert np.abs(got).max() > 0.5
+
+
+def test_covariance_oracle_catches_the_textbook_mean(oracle):
+    # The reference divides the column sums by the constant 3214212.01, not by
+    # the number of observations; an oracle that used the textbook mean would
+    # fail the reference's own rule.
+    n = 64
+    d, s = oracle.covariance_reference(n)
+    textbook = _stats_f64(d[1:, 1:].astype(np.float64), float(n), False)
+    assert oracle.percent_diff_count(textbook, s[1:, 1:]) > 0
