@@ -62,6 +62,42 @@ def time_steps(torch, h, A, B, C, prec, steps, warmup, alpha, beta):
     return start.elapsed_time(stop), wall, kernels
 
 
+def calibrate(torch, h, reps=5):
+    """Measured ceilings: the MFMA burn loop (BF16 and FP32 instructions) and a
+    2 GiB device copy through the library's own calibrators, average kernel
+    time over `reps` launches each."""
+    waves = h.mfma_burn_waves()
+    out = torch.empty(waves, device="cuda")
+    res = {}
+    for prec, kernel in (("bf16", "mfma_burn_bf16"), ("fp32", "mfma_burn_f32")):
+        h.mfma_burn(out, 4096, prec=prec)
+        torch.cuda.synchronize()
+        h.timing_begin()
+        for _ in range(reps):
+            flop, expected = h.mfma_burn(out, 4096, prec=prec)
+        torch.cuda.synchronize()
+        launches, ms = h.timing_end()[kernel]
+        if not torch.all(out == expected).item():
+            raise RuntimeError(f"{kernel}: checksum mismatch")
+        res[f"mfma_{prec}_tflops"] = round(flop / (ms / launches / 1e3) / 1e12, 1)
+    nbytes = 2 << 30
+    src = torch.ones(nbytes // 4, device="cuda")
+    dst = torch.empty_like(src)
+    h.hbm_copy(src, dst)
+    torch.cuda.synchronize()
+    h.timing_begin()
+    for _ in range(reps):
+        h.hbm_copy(src, dst)
+    torch.cuda.synchronize()
+    launches, ms = h.timing_end()["hbm_copy"]
+    if not torch.equal(src, dst):
+        raise RuntimeError("hbm_copy: copy mismatch")
+    res["hbm_copy_gbs"] = round(2 * nbytes / (ms / launches / 1e3) / 1e9, 1)
+    res["definition"] = ("mfx_mfma_burn: 2 waves/SIMD x 8 independent 32x32 MFMAs x 4096 iterations; "
+                         "mfx_hbm_copy: 2 GiB read + 2 GiB written, 128-bit accesses")
+    return res
+
+
 def grouped_workload(torch, mfx, h, steps, warmup, seed=1):
     """512 independent problems (M, N, K log-uniform on [64, 2048], multiples of
     16, from `seed`) plus an 8-long chain of 2048 x 2048 matrices, as one
@@ -293,6 +329,12 @@ def main():
                      "kernel": "gemm_split_bf16_mfma", "kernel_ms": round(k_avg, 4),
                      "achieved_definition": "3 BF16 MFMA products x 2N^3 flop / average kernel duration"},
     }
+    log("calibrators ...")
+    cal = calibrate(torch, h)
+    cal["split_kernel_frac_of_measured_bf16"] = round(achieved / cal["mfma_bf16_tflops"], 4)
+    cal["fp32_kernel_frac_of_measured_fp32"] = round(fp32_kernel_tflops / cal["mfma_fp32_tflops"], 4)
+    out["measured_ceilings"] = cal
+    log(f"measured ceilings: {cal}")
     if not args.no_grouped:
         g = grouped_workload(torch, mfx, h, max(3, args.steps), max(1, args.warmup))
         g["frac_of_large_tile"] = round(g["tflops"] / value, 3)

@@ -246,6 +246,28 @@ mfx_status mfx_timing_begin(mfx_handle handle);
 mfx_status mfx_timing_end(mfx_handle handle, int32_t cap, const char** names,
     int64_t* launches, double* total_ms, int32_t* n_out);
 
+/* ---- roofline calibrators ---------------------------------------------------- */
+
+/* Number of waves mfx_mfma_burn runs (two per SIMD on every CU). */
+int64_t mfx_mfma_burn_waves(mfx_handle handle);
+/*
+ * Register-only MFMA loop of `prec`'s matrix instruction (FP32: 32x32x2,
+ * split / single BF16: 32x32x16, FP8: scaled 32x32x64 e4m3): every wave runs
+ * `iterations` x 8 independent MFMAs.  out[w] receives wave w's checksum,
+ * which must equal *expected; *flop is the total work, so flop / kernel time
+ * is the measured matrix-core ceiling.  Follows micro/arith.cpp:41-68 (a
+ * known-result arithmetic loop) on the matrix cores.
+ */
+mfx_status mfx_mfma_burn(mfx_handle handle, mfx_precision prec, int64_t iterations, float* out,
+    int64_t out_len, double* flop, float* expected, mfx_stream stream);
+/*
+ * dst = src for `bytes` bytes (16-byte aligned buffers, not overlapping),
+ * with 128-bit loads and stores: 2 * bytes / kernel time is the measured HBM
+ * ceiling.
+ * Follows micro/DRAM.cpp:47-62.
+ */
+mfx_status mfx_hbm_copy(mfx_handle handle, const void* src, void* dst, int64_t bytes, mfx_stream stream);
+
 /* ---- autotuner -------------------------------------------------------------- */
 
 /*

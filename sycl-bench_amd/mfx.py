@@ -68,6 +68,9 @@ SIGNATURES = {
     "mfx_force_grouped_policy": (_ST, [_V, _I32]),
     "mfx_covariance": (_ST, [_V, _ST, _I64, _I64, _V, _I64, _F, _V, _V, _I64, _V, _I64, _V]),
     "mfx_correlation": (_ST, [_V, _ST, _I64, _I64, _V, _I64, _D, _D, _V, _V, _V, _I64, _V, _I64, _V]),
+    "mfx_mfma_burn_waves": (_I64, [_V]),
+    "mfx_mfma_burn": (_ST, [_V, _ST, _I64, _V, _I64, ctypes.POINTER(_D), ctypes.POINTER(_F), _V]),
+    "mfx_hbm_copy": (_ST, [_V, _V, _V, _I64, _V]),
     "mfx_gemm_f64": (_ST, [_V, _ST, _ST, _I64, _I64, _I64, _D, _V, _I64, _V, _I64, _D, _V, _I64, _V]),
     "mfx_convert_f32_f64": (_ST, [_V, _I64, _I64, _V, _I64, _V, _I64, _V]),
     "mfx_rel_frobenius": (_ST, [_V, _ST, _I64, _I64, _V, _I64, _V, _I64, ctypes.POINTER(_D), _V]),
@@ -188,6 +191,21 @@ class Handle:
         _check(self.lib.mfx_correlation(self.h, prec_id(prec), m, n, _ptr(data), _ld(data), n_const, eps, _ptr(mean),
                                         _ptr(stddev), _ptr(centered), _ld(centered), _ptr(symmat), _ld(symmat),
                                         _stream(stream)), "mfx_correlation")
+
+    def mfma_burn(self, out, iterations, prec="split-bf16", stream=None):
+        """Register-only MFMA loop; returns (flop, expected per-wave checksum)."""
+        flop, expected = ctypes.c_double(), ctypes.c_float()
+        _check(self.lib.mfx_mfma_burn(self.h, prec_id(prec), iterations, _ptr(out), out.numel(), ctypes.byref(flop),
+                                      ctypes.byref(expected), _stream(stream)), "mfx_mfma_burn")
+        return flop.value, expected.value
+
+    def mfma_burn_waves(self):
+        return self.lib.mfx_mfma_burn_waves(self.h)
+
+    def hbm_copy(self, src, dst, stream=None):
+        nbytes = src.numel() * src.element_size()
+        assert dst.numel() * dst.element_size() >= nbytes
+        _check(self.lib.mfx_hbm_copy(self.h, _ptr(src), _ptr(dst), nbytes, _stream(stream)), "mfx_hbm_copy")
 
     def gemm_f64(self, A, B, C, alpha=1.0, beta=0.0, transA=False, transB=False, stream=None):
         m, n = C.shape
