@@ -58,19 +58,13 @@ __global__ __launch_bounds__(256) void mfma_burn_kernel(uint32_t bits, int iters
   if((threadIdx.x & 63) == 0) out[(blockIdx.x * blockDim.x + threadIdx.x) >> 6] = s;
 }
 
-// 128-bit grid-stride copy of n chunks, four in flight per thread; the
-// last `tail` (< 16) bytes after them are copied by the first thread.
+// 128-bit copy of n chunks, one per thread over a grid that covers them all
+// (measured faster on MI355X than grid-stride loops with several chunks in
+// flight per thread: 6.25 against 4.4-5.0 TB/s for 2 GiB); the last `tail`
+// (< 16) bytes after them are copied by the first thread.
 __global__ __launch_bounds__(256) void hbm_copy_kernel(const f32x4_t* src, f32x4_t* dst, int64_t n, int tail) {
-  const int64_t stride = (int64_t)gridDim.x * 256;
-  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  for(; i + 3 * stride < n; i += 4 * stride) {
-    const f32x4_t a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-    dst[i] = a;
-    dst[i + stride] = b;
-    dst[i + 2 * stride] = c;
-    dst[i + 3 * stride] = d;
-  }
-  for(; i < n; i += stride) dst[i] = src[i];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if(i < n) dst[i] = src[i];
   if(tail && blockIdx.x == 0 && threadIdx.x == 0) {
     const unsigned char* s8 = reinterpret_cast<const unsigned char*>(src + n);
     unsigned char* d8 = reinterpret_cast<unsigned char*>(dst + n);
@@ -133,7 +127,8 @@ mfx_status mfx_hbm_copy(mfx_handle handle, const void* src, void* dst, int64_t b
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int64_t n = bytes / 16;
   const int tail = (int)(bytes % 16);
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256 * 4), (int64_t)std::max(1, h->cu_count) * 16));
+  if(n > (int64_t)0x7fffffff * 256) return invalid("copy larger than (2^31 - 1) * 256 chunks");
+  const unsigned grid = (unsigned)std::max<int64_t>(1, ceil_div(n, 256));
   return timed_launch(h, "hbm_copy", s, [&] {
     hbm_copy_kernel<<<grid, 256, 0, s>>>(static_cast<const f32x4_t*>(src), static_cast<f32x4_t*>(dst), n, tail);
   });

@@ -2,7 +2,7 @@
 // measured against.  micro/DRAM.cpp:27-96 with one-dimensional buffers: copy
 // --size^3 bytes of 33 from one device buffer to another, report
 // 2 x the copied GiB (read + write), verify that every element is 33.  The
-// copy is mfx_hbm_copy (128-bit loads and stores, four in flight per
+// copy is mfx_hbm_copy (128-bit loads and stores, one chunk per
 // thread).  The reference's 2-D and 3-D variants copy the same contiguous
 // bytes through a different index space and have no counterpart here.
 #include <string>
