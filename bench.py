@@ -63,23 +63,26 @@ def time_steps(torch, h, A, B, C, prec, steps, warmup, alpha, beta):
 
 
 def calibrate(torch, h, reps=5):
-    """Measured ceilings: the MFMA burn loop (BF16 and FP32 instructions) and a
-    2 GiB device copy through the library's own calibrators, average kernel
-    time over `reps` launches each."""
+    """Measured ceilings: the MFMA burn loop (BF16 and FP32 instructions, with
+    constant and with random operands) and a 2 GiB device copy through the
+    library's own calibrators, average kernel time over `reps` launches each."""
     waves = h.mfma_burn_waves()
     out = torch.empty(waves, device="cuda")
     res = {}
     for prec, kernel in (("bf16", "mfma_burn_bf16"), ("fp32", "mfma_burn_f32")):
-        h.mfma_burn(out, 4096, prec=prec)
-        torch.cuda.synchronize()
-        h.timing_begin()
-        for _ in range(reps):
-            flop, expected = h.mfma_burn(out, 4096, prec=prec)
-        torch.cuda.synchronize()
-        launches, ms = h.timing_end()[kernel]
-        if not torch.all(out == expected).item():
-            raise RuntimeError(f"{kernel}: checksum mismatch")
-        res[f"mfma_{prec}_tflops"] = round(flop / (ms / launches / 1e3) / 1e12, 1)
+        for rnd in (False, True):
+            iters = 1024 if rnd else 4096
+            name = kernel + ("_random" if rnd else "")
+            h.mfma_burn(out, iters, prec=prec, random_operands=rnd)
+            torch.cuda.synchronize()
+            h.timing_begin()
+            for _ in range(reps):
+                flop, expected = h.mfma_burn(out, iters, prec=prec, random_operands=rnd)
+            torch.cuda.synchronize()
+            launches, ms = h.timing_end()[name]
+            if not torch.all(out == expected).item():
+                raise RuntimeError(f"{name}: checksum mismatch")
+            res[f"mfma_{prec}{'_random' if rnd else ''}_tflops"] = round(flop / (ms / launches / 1e3) / 1e12, 1)
     nbytes = 2 << 30
     src = torch.ones(nbytes // 4, device="cuda")
     dst = torch.empty_like(src)
@@ -93,8 +96,9 @@ def calibrate(torch, h, reps=5):
     if not torch.equal(src, dst):
         raise RuntimeError("hbm_copy: copy mismatch")
     res["hbm_copy_gbs"] = round(2 * nbytes / (ms / launches / 1e3) / 1e9, 1)
-    res["definition"] = ("mfx_mfma_burn: 2 waves/SIMD x 8 independent 32x32 MFMAs x 4096 iterations; "
-                         "mfx_hbm_copy: 2 GiB read + 2 GiB written, 128-bit accesses")
+    res["definition"] = ("mfx_mfma_burn: 2 waves/SIMD x 8 independent 32x32 MFMAs per step; constant operands "
+                         "(4096 steps, minimal switching power) and random operands (4097 steps that cancel "
+                         "exactly, a contraction's power draw); mfx_hbm_copy: 2 GiB read + 2 GiB written")
     return res
 
 
@@ -332,7 +336,9 @@ def main():
     log("calibrators ...")
     cal = calibrate(torch, h)
     cal["split_kernel_frac_of_measured_bf16"] = round(achieved / cal["mfma_bf16_tflops"], 4)
+    cal["split_kernel_frac_of_measured_bf16_random"] = round(achieved / cal["mfma_bf16_random_tflops"], 4)
     cal["fp32_kernel_frac_of_measured_fp32"] = round(fp32_kernel_tflops / cal["mfma_fp32_tflops"], 4)
+    cal["fp32_kernel_frac_of_measured_fp32_random"] = round(fp32_kernel_tflops / cal["mfma_fp32_random_tflops"], 4)
     out["measured_ceilings"] = cal
     log(f"measured ceilings: {cal}")
     if not args.no_grouped:

@@ -252,14 +252,19 @@ mfx_status mfx_timing_end(mfx_handle handle, int32_t cap, const char** names,
 int64_t mfx_mfma_burn_waves(mfx_handle handle);
 /*
  * Register-only MFMA loop of `prec`'s matrix instruction (FP32: 32x32x2,
- * split / single BF16: 32x32x16, FP8: scaled 32x32x64 e4m3): every wave runs
- * `iterations` x 8 independent MFMAs.  out[w] receives wave w's checksum,
+ * split / single BF16: 32x32x16, FP8: scaled 32x32x64 e4m3) on two waves per
+ * SIMD, 8 independent accumulators per wave.  random_operands = 0: constant
+ * power-of-two operands, `iterations` MFMAs per accumulator (the matrix
+ * cores' upper bound: their switching power, and so the clock drop, is
+ * minimal).  random_operands = 1 (FP32 and BF16 only): per-lane random
+ * operands, 4 * iterations MFMAs per accumulator that cancel exactly, plus
+ * one constant one (the power draw of a real contraction).  out[w] receives wave w's checksum,
  * which must equal *expected; *flop is the total work, so flop / kernel time
  * is the measured matrix-core ceiling.  Follows micro/arith.cpp:41-68 (a
  * known-result arithmetic loop) on the matrix cores.
  */
-mfx_status mfx_mfma_burn(mfx_handle handle, mfx_precision prec, int64_t iterations, float* out,
-    int64_t out_len, double* flop, float* expected, mfx_stream stream);
+mfx_status mfx_mfma_burn(mfx_handle handle, mfx_precision prec, int64_t iterations, int32_t random_operands,
+    float* out, int64_t out_len, double* flop, float* expected, mfx_stream stream);
 /*
  * dst = src for `bytes` bytes (16-byte aligned buffers, not overlapping),
  * with 128-bit loads and stores: 2 * bytes / kernel time is the measured HBM

@@ -69,7 +69,7 @@ SIGNATURES = {
     "mfx_covariance": (_ST, [_V, _ST, _I64, _I64, _V, _I64, _F, _V, _V, _I64, _V, _I64, _V]),
     "mfx_correlation": (_ST, [_V, _ST, _I64, _I64, _V, _I64, _D, _D, _V, _V, _V, _I64, _V, _I64, _V]),
     "mfx_mfma_burn_waves": (_I64, [_V]),
-    "mfx_mfma_burn": (_ST, [_V, _ST, _I64, _V, _I64, ctypes.POINTER(_D), ctypes.POINTER(_F), _V]),
+    "mfx_mfma_burn": (_ST, [_V, _ST, _I64, _I32, _V, _I64, ctypes.POINTER(_D), ctypes.POINTER(_F), _V]),
     "mfx_hbm_copy": (_ST, [_V, _V, _V, _I64, _V]),
     "mfx_gemm_f64": (_ST, [_V, _ST, _ST, _I64, _I64, _I64, _D, _V, _I64, _V, _I64, _D, _V, _I64, _V]),
     "mfx_convert_f32_f64": (_ST, [_V, _I64, _I64, _V, _I64, _V, _I64, _V]),
@@ -192,11 +192,12 @@ class Handle:
                                         _ptr(stddev), _ptr(centered), _ld(centered), _ptr(symmat), _ld(symmat),
                                         _stream(stream)), "mfx_correlation")
 
-    def mfma_burn(self, out, iterations, prec="split-bf16", stream=None):
+    def mfma_burn(self, out, iterations, prec="split-bf16", random_operands=False, stream=None):
         """Register-only MFMA loop; returns (flop, expected per-wave checksum)."""
         flop, expected = ctypes.c_double(), ctypes.c_float()
-        _check(self.lib.mfx_mfma_burn(self.h, prec_id(prec), iterations, _ptr(out), out.numel(), ctypes.byref(flop),
-                                      ctypes.byref(expected), _stream(stream)), "mfx_mfma_burn")
+        _check(self.lib.mfx_mfma_burn(self.h, prec_id(prec), iterations, int(bool(random_operands)), _ptr(out),
+                                      out.numel(), ctypes.byref(flop), ctypes.byref(expected), _stream(stream)),
+               "mfx_mfma_burn")
         return flop.value, expected.value
 
     def mfma_burn_waves(self):

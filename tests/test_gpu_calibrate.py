@@ -20,14 +20,22 @@ BURN = {"fp32": (2, 1 / 32), "bf16": (16, 1 / 4), "split-bf16": (16, 1 / 4), "fp
 
 @pytest.mark.parametrize("prec", list(BURN))
 @pytest.mark.parametrize("iters", [1, 37, 256])
-def test_mfma_burn_checksum_and_flop(handle, prec, iters):
+@pytest.mark.parametrize("random_operands", [False, True])
+def test_mfma_burn_checksum_and_flop(handle, prec, iters, random_operands):
     waves = handle.mfma_burn_waves()
     assert waves == 8 * torch.cuda.get_device_properties(0).multi_processor_count
     out = torch.full((waves,), float("nan"), device=DEV)
-    flop, expected = handle.mfma_burn(out, iters, prec=prec)
+    if random_operands and prec.startswith("fp8"):
+        with pytest.raises(RuntimeError, match="FP32 and BF16 only"):
+            handle.mfma_burn(out, iters, prec=prec, random_operands=True)
+        return
+    flop, expected = handle.mfma_burn(out, iters, prec=prec, random_operands=random_operands)
     k, per = BURN[prec]
-    assert expected == 8 * iters * per
-    assert flop == waves * 8 * iters * 2 * 32 * 32 * k
+    # random: 4 cancelling MFMAs per iteration and one constant one, so every
+    # accumulator ends at exactly one constant product
+    mfmas = 4 * iters + 1 if random_operands else iters
+    assert expected == 8 * (1 if random_operands else iters) * per
+    assert flop == waves * 8 * mfmas * 2 * 32 * 32 * k
     torch.cuda.synchronize()
     assert torch.all(out == expected).item(), out.unique()
 
@@ -61,10 +69,17 @@ def test_hbm_copy_rejects_misaligned(handle):
         handle.hbm_copy(src[4:], dst)
 
 
+@pytest.mark.parametrize("operands", ["constant", "random"])
 @pytest.mark.parametrize("prec,name", [("fp32", "MicroBench_MFMA_fp32"), ("bf16", "MicroBench_MFMA_bf16"),
                                        ("split-bf16", "MicroBench_MFMA_bf16"), ("fp8-block", "MicroBench_MFMA_fp8block")])
-def test_mfma_peak_program(prec, name):
-    r, res = bench("mfma_peak", "--size=512", "--num-runs=2", "--queue-profiling", f"--precision={prec}")
+def test_mfma_peak_program(prec, name, operands):
+    r, res = bench("mfma_peak", "--size=512", "--num-runs=2", "--queue-profiling", f"--precision={prec}",
+                   f"--operands={operands}")
+    if operands == "random":
+        name += "_random"
+        if prec.startswith("fp8"):
+            assert r.returncode != 0 and "FP32 and BF16 only" in r.stderr + r.stdout
+            return
     assert r.returncode == 0, r.stderr
     assert res["_names"] == [name]
     assert res["Verification"] == "PASS"
