@@ -22,14 +22,26 @@ struct PackParams {
   u32x4* dst;
 };
 
+// One lane-chunk per thread; a non-k-major source goes through the wave's
+// LDS tile (load_chunk8_staged) so its reads are 128-byte row segments, with
+// the (blk, kb) pairs walked kb-major so that neighbouring waves read
+// neighbouring segments of the same source rows.
 template <int NHL>
 __global__ __launch_bounds__(256) void pack_bf16x_kernel(const PackParams p) {
+  __shared__ float tiles[4][STAGED_TILE_FLOATS];
   const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t total = (int64_t)p.blocks * p.kbs * 64;
-  if(gid >= total) return;
+  const bool active = gid < total;
   const int lane = (int)(gid & 63);
   const int64_t t = gid >> 6;
-  pack_chunk<NHL>(p.s, (int)(t / p.kbs), (int)(t % p.kbs), lane, p.kbs, p.dst);
+  if(p.s.kmajor) {
+    if(active) pack_chunk<NHL>(p.s, (int)(t / p.kbs), (int)(t % p.kbs), lane, p.kbs, p.dst);
+    return;
+  }
+  const int blk = (int)(t % p.blocks), kb = (int)(t / p.blocks);
+  float v[8];
+  load_chunk8_staged(p.s, blk, kb, lane, active, v, tiles[threadIdx.x >> 6]);
+  if(active) store_chunk<NHL>(p.dst, blk, kb, lane, p.kbs, v);
 }
 
 struct BfParams {
@@ -121,7 +133,8 @@ PackSrc make_pack_src(const float* src, int64_t ld, int rows, int K, bool kmajor
   s.rows = rows;
   s.K = K;
   s.kmajor = kmajor;
-  s.vec = kmajor && (reinterpret_cast<uintptr_t>(src) & 15u) == 0 && ld % 4 == 0;
+  // 128-bit loads along the contiguous dimension (k if kmajor, rows if not)
+  s.vec = (reinterpret_cast<uintptr_t>(src) & 15u) == 0 && ld % 4 == 0;
   return s;
 }
 

@@ -39,9 +39,13 @@ __device__ void pack_item(const GProblem& P, bool is_a, int blk, int slice) {
   }
 }
 
-// Pre-pass: block b packs chunks of the job whose block range holds b.
+// Pre-pass: block b packs chunks of the job whose block range holds b (jobs
+// start on 256-chunk blocks, so each wave's 64 chunks are one (blk, kb)
+// pair, as load_chunk8_staged needs; a non-k-major source walks the pairs
+// kb-major, neighbouring waves on neighbouring row segments).
 template <int NHL>
 __global__ __launch_bounds__(256) void pack_jobs_kernel(const PackJob* jobs, int njobs) {
+  __shared__ float tiles[4][STAGED_TILE_FLOATS];
   int lo = 0, hi = njobs - 1;
   const int b = blockIdx.x;
   while(lo < hi) {
@@ -51,10 +55,18 @@ __global__ __launch_bounds__(256) void pack_jobs_kernel(const PackJob* jobs, int
   }
   const PackJob& j = jobs[lo];
   const int64_t c = (int64_t)(b - j.first_block) * 256 + threadIdx.x;
-  if(c >= j.chunks) return;
+  const bool active = c < j.chunks;
   const int lane = (int)(c & 63);
   const int64_t t = c >> 6;
-  pack_chunk<NHL>(j.src, (int)(t / j.kbs), (int)(t % j.kbs), lane, j.kbs, j.dst);
+  if(j.src.kmajor) {
+    if(active) pack_chunk<NHL>(j.src, (int)(t / j.kbs), (int)(t % j.kbs), lane, j.kbs, j.dst);
+    return;
+  }
+  const int nblk = (int)(j.chunks / ((int64_t)j.kbs * 64));
+  const int blk = (int)(t % nblk), kb = (int)(t / nblk);
+  float v[8];
+  load_chunk8_staged(j.src, blk, kb, lane, active, v, tiles[threadIdx.x >> 6]);
+  if(active) store_chunk<NHL>(j.dst, blk, kb, lane, j.kbs, v);
 }
 
 template <int NHL, int C>

@@ -100,6 +100,51 @@ __device__ __forceinline__ void load_chunk8(const PackSrc& s, int blk, int kb, i
   }
 }
 
+// LDS floats per wave for load_chunk8_staged: 16 k rows of 32 floats, rows
+// 8..15 shifted by 32 floats so the two half-waves' column reads fall in
+// different banks.
+constexpr int STAGED_TILE_FLOATS = 16 * 32 + 32;
+
+// load_chunk8 for a non-k-major source (element (r, k) at src[k*ld + r]),
+// staged through LDS: the 64 lanes of one wave hold the 64 lane-chunks of one
+// (blk, kb) pair; together they load its 16 x 32 (k x row) source tile as
+// 16 row segments of 128 bytes (128-bit loads when s.vec), then each lane
+// reads its 8 k values of one row from LDS.  `tile` is the wave's
+// STAGED_TILE_FLOATS area.  Contains a workgroup barrier: every thread of the
+// block calls it, `active` false for threads without a chunk (whose wave may
+// still hold active lanes).
+__device__ __forceinline__ void load_chunk8_staged(const PackSrc& s, int blk, int kb, int lane, bool active,
+    float (&v)[8], float* tile) {
+  const int r0 = blk * 32, k0 = kb * 16;
+  if(active) {
+#pragma unroll
+    for(int h = 0; h < 2; ++h) {
+      const int q = lane + h * 64;  // 128 float4 slots: 16 k rows x 8
+      const int k = q >> 3, r = (q & 7) * 4;
+      const int gk = k0 + k, gr = r0 + r;
+      float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if(gk < s.K) {
+        const float* p = s.src + (int64_t)gk * s.ld + gr;
+        if(s.vec && gr + 4 <= s.rows) {
+          x = *reinterpret_cast<const float4*>(p);
+        } else {
+          if(gr < s.rows) x.x = p[0];
+          if(gr + 1 < s.rows) x.y = p[1];
+          if(gr + 2 < s.rows) x.z = p[2];
+          if(gr + 3 < s.rows) x.w = p[3];
+        }
+      }
+      *reinterpret_cast<float4*>(tile + k * 32 + (k >= 8 ? 32 : 0) + r) = x;
+    }
+  }
+  __syncthreads();
+  if(active) {
+    const int kh = (lane >> 5) * 8, r = lane & 31;
+#pragma unroll
+    for(int e = 0; e < 8; ++e) v[e] = tile[(kh + e) * 32 + (kh ? 32 : 0) + r];
+  }
+}
+
 // Splits a loaded lane-chunk and stores it at its fragment-order position in
 // the packed operand `dst` (kbs = number of 16-wide k blocks).
 template <int NHL>
