@@ -275,7 +275,7 @@ def main():
     R = C0.double()
     h.gemm_f64(A.double(), B.double(), R, alpha=alpha, beta=beta)
     errs = {}
-    for prec in ("split-bf16", "fp32"):
+    for prec in ("split-bf16", "fp32", "bf16", "fp8-block"):
         C = C0.clone()
         h.gemm(A, B, C, alpha=alpha, beta=beta, prec=prec)
         errs[prec] = h.rel_frobenius(C, R)
@@ -285,7 +285,7 @@ def main():
 
     flop = 2.0 * n ** 3
     res = {}
-    for prec in ("fp32", "split-bf16"):
+    for prec in ("fp32", "split-bf16", "bf16", "fp8-block"):
         C = C0.clone()
         ms, wall, kernels = time_steps(torch, h, A, B, C, prec, args.steps, args.warmup, alpha, beta)
         res[prec] = (ms, wall, kernels)
@@ -303,6 +303,14 @@ def main():
     f_launch, f_ms = res["fp32"][2]["gemm_f32_mfma"]
     fp32_kernel_tflops = flop / (f_ms / f_launch / 1e3) / 1e12
     pack_ms = sum(v[1] for k, v in kernels.items() if k.startswith("pack")) / args.steps
+    # the other precisions of the same call, reported beside the headline
+    others = {}
+    for prec, kname in (("bf16", "gemm_bf16_mfma"), ("fp8-block", "gemm_fp8_block_mfma")):
+        p_ms, _, p_kernels = res[prec]
+        k_launch, k_ms = p_kernels[kname]
+        others[prec] = {"tflops": round(flop / (p_ms / args.steps / 1e3) / 1e12, 1),
+                        "kernel_tflops": round(flop / (k_ms / k_launch / 1e3) / 1e12, 1),
+                        "kernel": kname, "rel_err": errs[prec]}
 
     out = {
         "metric": METRIC,
@@ -328,6 +336,7 @@ def main():
         "split_vs_fp32": round(value / fp32_tflops, 3),
         "split_fp32_equiv_frac": round(value / PEAK_FP32, 3),
         "pack_ms_per_step": round(pack_ms, 4),
+        "other_precisions": others,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_BF16, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_BF16, 4), "traffic": measured_traffic(n, "gemm_split_bf16_mfma"),
                      "kernel": "gemm_split_bf16_mfma", "kernel_ms": round(k_avg, 4),
