@@ -2,7 +2,7 @@
 
 One step = one mfx_gemm call (C = 1.5*A*B + 1.2*C, A/B/C seeded U[-1,1) and
 resident in HBM): the split-BF16 pack kernel plus the contraction kernel.
-Prints one JSON line (the driver's contract) with the split-BF16 throughput as
+Prints one JSON line with the split-BF16 throughput as
 `value`, the FP32-MFMA throughput beside it, the relative Frobenius errors of
 both against the GPU FP64 oracle, the roofline of the dominant kernel (timed
 with hipEvents on the launch stream inside the timed region) and the CPU

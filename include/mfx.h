@@ -31,7 +31,7 @@ extern "C" {
 typedef enum mfx_status {
   MFX_SUCCESS = 0,
   MFX_ERROR_INVALID_VALUE = 1,   /* bad size, pointer, enum or leading dimension */
-  MFX_ERROR_NOT_SUPPORTED = 2,   /* valid request that this build does not implement */
+  MFX_ERROR_NOT_SUPPORTED = 2,   /* valid request that the library does not implement */
   MFX_ERROR_HIP = 3,             /* HIP runtime error (see mfx_last_error) */
   MFX_ERROR_ALLOC = 4,           /* workspace allocation failed */
   MFX_ERROR_NO_DEVICE = 5,       /* no usable gfx950 device */
@@ -187,7 +187,7 @@ mfx_status mfx_graph_launch(mfx_graph graph, mfx_stream stream);
 mfx_status mfx_graph_status(mfx_graph graph, mfx_stream stream);
 mfx_status mfx_graph_destroy(mfx_graph graph);
 
-/* ---- FP64 oracle and error metrics (checkers, not part of any product) ---- */
+/* ---- FP64 oracle and error metrics (scoring only, not part of any product) ---- */
 
 /* C = alpha * op(A) * op(B) + beta * C entirely in FP64 (beta == 0: C not read). */
 mfx_status mfx_gemm_f64(mfx_handle handle, mfx_op transA, mfx_op transB, int64_t m,

@@ -1,7 +1,7 @@
 /*
  * oracle.c -- CPU oracle for the dense-contraction path.  TEST INFRASTRUCTURE
  * ONLY: it is loaded by tests/, by __graft_entry__.smoke() and by the
- * cpu_baseline leg of bench.py as the checker, never by the library or the
+ * cpu_baseline leg of bench.py to compare against, never by the library or the
  * benchmark executables.
  *
  * Contents:
@@ -18,7 +18,7 @@
  *
  * Built strictly (no -ffast-math, -ffp-contract=off) so every result is
  * reproducible; the reference itself builds with -ffast-math
- * (CMakeLists.txt:17), which is why parity with it is judged by its own
+ * (CMakeLists.txt:17), which is why parity with it is measured by its own
  * percentDiff threshold, not bit equality.
  */
 #include <math.h>

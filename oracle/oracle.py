@@ -1,7 +1,7 @@
 """ctypes view of the CPU oracle (oracle.c) and CPU baseline (cpu_baseline.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
-bench.py's cpu_baseline leg, as the checker.  The library and the benchmark
+bench.py's cpu_baseline leg, to compare against.  The library and the benchmark
 executables never load it.
 
 Every wrapper takes and returns numpy arrays; each C function cites the

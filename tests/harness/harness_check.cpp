@@ -1,4 +1,4 @@
-// CPU-only driver for the harness tests: fake device and benchmarks that
+// CPU-only program for the harness tests: fake device and benchmarks that
 // exercise BenchmarkApp / BenchmarkManager / TimeMetricsProcessor /
 // ResultConsumer without a GPU.  The scenario comes from HARNESS_SCENARIO.
 #include <chrono>

@@ -1,4 +1,4 @@
-"""The suite driver over the real programs on the GPU: the gpu-smoke profile
+"""The suite runner over the real programs on the GPU: the gpu-smoke profile
 (every program once at --size=256, two runs each) writes one results file in
 which every benchmark verified, and the one-table merge holds every row."""
 import os

@@ -11,16 +11,16 @@ GOLDEN = open(os.path.join(ROOT, "tests", "golden", "reference_csv_header.txt"))
 
 
 @pytest.fixture(scope="module")
-def driver(tmp_path_factory):
+def harness_exe(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("harness") / "harness_check")
     subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", HARNESS,
                     os.path.join(ROOT, "tests", "harness", "harness_check.cpp"), "-o", out, "-lpthread"], check=True)
     return out
 
 
-def run(driver, scenario, *args, env=None):
+def run(harness_exe, scenario, *args, env=None):
     e = dict(os.environ, HARNESS_SCENARIO=scenario, **(env or {}))
-    return subprocess.run([driver, *args], capture_output=True, text=True, env=e)
+    return subprocess.run([harness_exe, *args], capture_output=True, text=True, env=e)
 
 
 def parse_stdout(text):
@@ -32,9 +32,9 @@ def parse_stdout(text):
     return res
 
 
-def test_csv_header_matches_reference(driver, tmp_path):
+def test_csv_header_matches_reference(harness_exe, tmp_path):
     csv = tmp_path / "out.csv"
-    r = run(driver, "plain", "--size=64", "--num-runs=3", f"--output={csv}")
+    r = run(harness_exe, "plain", "--size=64", "--num-runs=3", f"--output={csv}")
     assert r.returncode == 0, r.stderr
     lines = csv.read_text().splitlines()
     assert lines[0] == GOLDEN
@@ -65,28 +65,28 @@ def _split_csv(line):
     return out
 
 
-def test_csv_appends_one_header_per_flush(driver, tmp_path):
+def test_csv_appends_one_header_per_flush(harness_exe, tmp_path):
     csv = tmp_path / "out.csv"
     for _ in range(2):
-        assert run(driver, "plain", "--num-runs=1", f"--output={csv}").returncode == 0
+        assert run(harness_exe, "plain", "--num-runs=1", f"--output={csv}").returncode == 0
     lines = csv.read_text().splitlines()
     assert lines[0] == lines[2] == GOLDEN and len(lines) == 4
 
 
-def test_queue_profiling_fills_kernel_times(driver):
-    r = run(driver, "timed", "--num-runs=2", "--queue-profiling")
+def test_queue_profiling_fills_kernel_times(harness_exe):
+    r = run(harness_exe, "timed", "--num-runs=2", "--queue-profiling")
     res = parse_stdout(r.stdout)
     assert res["kernel-time-mean"] == "0.000100 [s]"
     assert res["submit-time-min"] == "0.000001 [s]"
     assert res["throughput-metric"] == "1000000000.000000 [FLOP]"
     assert res["kernel-time-throughput"].endswith("[FLOP/s]")
     assert res["Verification"] == "N/A"  # no verify() method
-    off = parse_stdout(run(driver, "timed", "--num-runs=2").stdout)
+    off = parse_stdout(run(harness_exe, "timed", "--num-runs=2").stdout)
     assert off["kernel-time-mean"] == "N/A" and off["system-time-median"] == "N/A"
 
 
-def test_statistics_values(driver):
-    res = parse_stdout(run(driver, "stats").stdout)
+def test_statistics_values(harness_exe):
+    res = parse_stdout(run(harness_exe, "stats").stdout)
     # samples 1,2,3,4,10 ms: mean 4, sample stddev sqrt(50/4), median element 2 of the sorted list
     assert res["run-time-mean"] == "0.004000 [s]"
     assert res["run-time-median"] == "0.003000 [s]"
@@ -96,65 +96,65 @@ def test_statistics_values(driver):
     assert res["run-time-throughput"] == "1000000000000.000000 [FLOP/s]"  # 1e9 FLOP / min
     assert res["kernel-time-mean"] == "N/A"
     # --warmup-run drops the first sample (4 ms): 1,2,3,10 -> mean 4, median 3
-    w = parse_stdout(run(driver, "stats", env={"HARNESS_WARMUP": "1"}).stdout)
+    w = parse_stdout(run(harness_exe, "stats", env={"HARNESS_WARMUP": "1"}).stdout)
     assert w["run-time-samples"] == '"0.001000 0.002000 0.003000 0.010000"'
 
 
-def test_verification_failure_stops_runs(driver):
-    r = run(driver, "plain", "--num-runs=5", "--fail-at=2")
+def test_verification_failure_stops_runs(harness_exe):
+    r = run(harness_exe, "plain", "--num-runs=5", "--fail-at=2")
     assert parse_stdout(r.stdout)["Verification"] == "FAIL"
     assert "runs=2 verifies=2" in r.stderr
     assert r.returncode == 1
 
 
-def test_no_verification_flag(driver):
-    r = run(driver, "plain", "--num-runs=2", "--no-verification")
+def test_no_verification_flag(harness_exe):
+    r = run(harness_exe, "plain", "--num-runs=2", "--no-verification")
     assert parse_stdout(r.stdout)["Verification"] == "N/A"
     assert "verifies=0" in r.stderr
-    r = run(driver, "plain", "--num-runs=2", "--verification-range=0,1,1")
+    r = run(harness_exe, "plain", "--num-runs=2", "--verification-range=0,1,1")
     assert parse_stdout(r.stdout)["Verification"] == "N/A"
 
 
-def test_verification_range_parses_by_element_count(driver):
-    r = run(driver, "plain", "--num-runs=1", "--verification-range=1,1,1", "--verification-begin=0,0")
+def test_verification_range_parses_by_element_count(harness_exe):
+    r = run(harness_exe, "plain", "--num-runs=1", "--verification-range=1,1,1", "--verification-begin=0,0")
     assert r.returncode == 0, r.stderr
     assert parse_stdout(r.stdout)["Verification"] == "PASS"
 
 
-def test_fresh_object_per_run(driver):
-    r = run(driver, "plain", "--num-runs=4")
+def test_fresh_object_per_run(harness_exe):
+    r = run(harness_exe, "plain", "--num-runs=4")
     assert "setups=4 runs=4 verifies=4" in r.stderr
 
 
-def test_throwing_benchmark_discards_row_and_continues(driver, tmp_path):
+def test_throwing_benchmark_discards_row_and_continues(harness_exe, tmp_path):
     csv = tmp_path / "out.csv"
-    r = run(driver, "throwing", "--num-runs=1", f"--output={csv}")
+    r = run(harness_exe, "throwing", "--num-runs=1", f"--output={csv}")
     assert "setup exploded" in r.stderr
     assert r.returncode == 1
     lines = csv.read_text().splitlines()
     assert [l.split(",")[0] for l in lines[1:]] == ["Plain"]
 
 
-def test_duplicate_names_rejected(driver):
-    r = run(driver, "duplicate", "--num-runs=1")
+def test_duplicate_names_rejected(harness_exe):
+    r = run(harness_exe, "duplicate", "--num-runs=1")
     assert "has already been run" in r.stderr and r.returncode == 1
     assert r.stdout.count("Results for Plain") == 1
 
 
-def test_bad_command_lines(driver):
-    r = run(driver, "plain", "--size=10", "--size=20")
+def test_bad_command_lines(harness_exe):
+    r = run(harness_exe, "plain", "--size=10", "--size=20")
     assert "several times" in r.stderr and r.returncode == 1
-    r = run(driver, "plain", "--size=abc")
+    r = run(harness_exe, "plain", "--size=abc")
     assert r.returncode == 1 and "--size" in r.stderr
-    r = run(driver, "plain", "--device=tpu")
+    r = run(harness_exe, "plain", "--device=tpu")
     assert r.returncode == 1 and "unknown device type" in r.stderr
-    r = run(driver, "plain", "--num-runs=0")
+    r = run(harness_exe, "plain", "--num-runs=0")
     assert r.returncode == 1
 
 
-def test_several_benchmarks_share_one_csv_header(driver, tmp_path):
+def test_several_benchmarks_share_one_csv_header(harness_exe, tmp_path):
     csv = tmp_path / "out.csv"
-    assert run(driver, "all", "--num-runs=1", f"--output={csv}").returncode == 0
+    assert run(harness_exe, "all", "--num-runs=1", f"--output={csv}").returncode == 0
     lines = csv.read_text().splitlines()
     headers = [l for l in lines if l.startswith("#")]
     assert all(h == GOLDEN for h in headers) and len(lines) == 6

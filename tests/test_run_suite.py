@@ -1,4 +1,4 @@
-"""The suite driver (sycl-bench_amd/run-suite) on the CPU, with stand-in
+"""The suite runner (sycl-bench_amd/run-suite) on the CPU, with stand-in
 programs: the reference's profile defaults and range helpers
 (bin/run-suite:10-24, :41-49), the argument order, the size-growth and
 abort-on-failure rules, the refusal to overwrite a results file, the sweep of
