@@ -234,8 +234,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-grouped", action="store_true")
     ap.add_argument("--profile-pass", action="store_true",
-                    help="short pass for counter collection: one split-BF16 and one FP32 gemm at --n, plus a "
-                         "1 GiB device copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters")
+                    help="short pass for counter collection: one split-BF16, FP32, single-BF16 and FP8-block gemm "
+                         "at --n, plus a 1 GiB device copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the dense-contraction path is built and measured on a single "
@@ -258,7 +258,7 @@ def main():
 
     if args.profile_pass:
         C = C0.clone()
-        for prec in ("split-bf16", "fp32"):
+        for prec in ("split-bf16", "fp32", "bf16", "fp8-block"):
             h.gemm(A, B, C, alpha=alpha, beta=beta, prec=prec)
             torch.cuda.synchronize()
             log(f"profile pass: {prec} gemm N={n} done")
