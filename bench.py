@@ -19,6 +19,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "GEMM TFLOP/s N=8192: split-BF16 vs FP32-MFMA at ≤1e-5 rel err; % MFMA peak"
 PEAK_BF16 = 2516.6  # TFLOP/s dense, MI355X
 PEAK_FP32 = 157.3
+PEAK_FP8 = 5033.2  # TFLOP/s dense, MI355X (twice the BF16 rate)
 
 
 def load_mfx():
@@ -308,8 +309,10 @@ def main():
     for prec, kname in (("bf16", "gemm_bf16_mfma"), ("fp8-block", "gemm_fp8_block_mfma")):
         p_ms, _, p_kernels = res[prec]
         k_launch, k_ms = p_kernels[kname]
+        k_tflops = flop / (k_ms / k_launch / 1e3) / 1e12
         others[prec] = {"tflops": round(flop / (p_ms / args.steps / 1e3) / 1e12, 1),
-                        "kernel_tflops": round(flop / (k_ms / k_launch / 1e3) / 1e12, 1),
+                        "kernel_tflops": round(k_tflops, 1),
+                        "kernel_frac_of_peak": round(k_tflops / (PEAK_BF16 if prec == "bf16" else PEAK_FP8), 4),
                         "kernel": kname, "rel_err": errs[prec]}
 
     out = {

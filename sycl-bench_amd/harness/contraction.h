@@ -7,7 +7,7 @@
 //   --init=polybench|uniform|lognormal   polybench = the reference's own
 //                                        closed-form inputs and constants
 //   --seed=<u64>  --alpha=<f>  --beta=<f>  --tolerance=<f>
-//   --metrics=extended   adds precision / error / TFLOP/s columns
+//   --metrics=extended   adds precision, init, seed, error and percentDiff columns
 // The reference names (Polybench_Gemm, ...) belong to fp32 + polybench; any
 // other combination appends "_<precision>" and/or "_<init>" so one program
 // can run several variants (duplicate names are rejected).
@@ -168,4 +168,6 @@ inline void contractionExtras(std::map<std::string, std::string>& out, const Con
   out["precision"] = o.precision_name;
   out["rel-frobenius-error"] = e.str();
   out["percent-diff-failures"] = std::to_string(c.percent_diff_failures);
+  out["init"] = o.init;
+  out["seed"] = o.reference_inputs() ? "N/A" : std::to_string(o.seed);
 }

@@ -64,6 +64,7 @@ def test_seeded_variants(exe, prec):
     r, res = bench(exe, "--size=512", "--num-runs=1", "--init=uniform", f"--precision={prec}", "--metrics=extended")
     assert r.returncode == 0, r.stderr
     assert res["Verification"] == "PASS", res
+    assert res["init"] == "uniform" and res["seed"] == "1"
     err = float(res["rel-frobenius-error"])
     assert err < {"fp32": 1e-5, "split-bf16": 2e-5, "bf16": 3e-2}[prec]
     suffix = "" if prec == "fp32" else "_" + prec.replace("-", "")
