@@ -85,7 +85,10 @@ __device__ __forceinline__ float e4m3_byte(uint32_t w) {
 // lanes).
 __global__ __launch_bounds__(256) void quantize_fp8_kernel(const QuantParams p) {
   __shared__ float sh[4];
-  const int rb = blockIdx.y, kb = blockIdx.x;
+  // Neighbouring workgroups take neighbouring blocks along the source's
+  // contiguous dimension (k if k-major, rows if not), so they read
+  // neighbouring segments of the same memory rows.
+  const int rb = p.kmajor ? blockIdx.y : blockIdx.x, kb = p.kmajor ? blockIdx.x : blockIdx.y;
   const int B = p.block;
   const int per_row = B / 16;
   const int chunks = B * per_row;  // <= 1024
@@ -376,7 +379,8 @@ mfx_status quantize(Handle* h, const float* src, int64_t ld, int rows, int K, bo
   p.e1 = e1;
   p.e2 = e2;
   p.block = 128;
-  dim3 grid((unsigned)kb128s, (unsigned)(rows_pad / 128));
+  const unsigned rbs = (unsigned)(rows_pad / 128);
+  const dim3 grid = kmajor ? dim3((unsigned)kb128s, rbs) : dim3(rbs, (unsigned)kb128s);
   return timed_launch(h, two ? "quantize_fp8_block2" : "quantize_fp8_block", s,
       [&] { quantize_fp8_kernel<<<grid, 256, 0, s>>>(p); });
 }
