@@ -132,3 +132,14 @@ def test_warmup_run_drops_first_sample():
 def test_device_cpu_is_rejected():
     r, _ = bench("gemm", "--size=64", "--device=cpu")
     assert r.returncode != 0 and "oracle" in r.stderr
+
+
+@pytest.mark.parametrize("exe", ["gemm", "2mm", "3mm", "syrk", "syr2k", "matmulchain", "covariance", "correlation"])
+def test_edge_sizes(exe):
+    # sizes below one MFMA block, one off a block edge on either side, and a seeded odd size
+    for n in (1, 2, 7, 31, 33, 65):
+        r, res = bench(exe, f"--size={n}", "--num-runs=1")
+        assert r.returncode == 0, (n, r.stderr)
+        assert res["Verification"] == "PASS", n
+    r, res = bench(exe, "--size=33", "--num-runs=1", "--init=uniform", "--precision=split-bf16")
+    assert r.returncode == 0 and res["Verification"] == "PASS", r.stderr
