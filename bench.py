@@ -193,6 +193,148 @@ def grouped_workload(torch, mfx, h, steps, warmup, seed=1):
             "chain_rel_err": chain_err, "sampled_rel_err_max": worst}
 
 
+def _time_ms(torch, fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def measure_configs(torch, mfx, h, reps=3):
+    """The other workloads at their full sizes, end to end per call (packing
+    included), with the relative Frobenius error against the GPU FP64 product:
+    gemm 1024 (FP32), 2mm / 3mm 4096 (per-call and grouped), syrk / syr2k 8192
+    lower triangle, an 8-long chain of 2048^2 (mfx_chain), FP8 block 16384
+    (sign x lognormal(0, 2) inputs, error on a 2048^2 block)."""
+    dev = "cuda"
+    out = {}
+
+    def seeded(shape, mid):
+        x = torch.empty(shape, device=dev)
+        h.fill_uniform(x, 1, mid)
+        return x
+
+    def rec(name, work, ms, err):
+        out[name] = {"ms": round(ms, 4), "tflops": round(work / (ms / 1e3) / 1e12, 1), "rel_err": err}
+        log(f"configs: {name}: {out[name]}")
+
+    # gemm N=1024, FP32 (alpha 1.5, beta 1.2)
+    n = 1024
+    A, B, C0 = seeded((n, n), 0), seeded((n, n), 1), seeded((n, n), 2)
+    R = C0.double()
+    h.gemm_f64(A.double(), B.double(), R, alpha=1.5, beta=1.2)
+    C = C0.clone()
+    ms = _time_ms(torch, lambda: h.gemm(A, B, C, alpha=1.5, beta=1.2, prec="fp32"), 20)
+    C = C0.clone()
+    h.gemm(A, B, C, alpha=1.5, beta=1.2, prec="fp32")
+    rec("gemm_1024_fp32", 2.0 * n ** 3, ms, h.rel_frobenius(C, R))
+
+    # 2mm D = 1.5 (A B) C + 1.2 D and 3mm G = (A B)(C D), 4096^2
+    n = 4096
+    A, B, Cm, D0 = (seeded((n, n), i) for i in range(4))
+    A64, B64, C64 = A.double(), B.double(), Cm.double()
+    T64 = torch.empty((n, n), device=dev, dtype=torch.float64)
+    h.gemm_f64(A64, B64, T64, alpha=1.5)
+    R2 = D0.double()
+    h.gemm_f64(T64, C64, R2, alpha=1.0, beta=1.2)
+    E64, F64, G64 = torch.empty_like(T64), torch.empty_like(T64), torch.empty_like(T64)
+    h.gemm_f64(A64, B64, E64)
+    h.gemm_f64(C64, D0.double(), F64)
+    h.gemm_f64(E64, F64, G64)
+    del T64, E64, F64, A64, B64, C64
+    T, D, E, F, G = (torch.empty((n, n), device=dev) for _ in range(5))
+    for prec in ("fp32", "split-bf16"):
+        for grouped in (False, True):
+            def mm2():
+                if grouped:
+                    h.grouped([mfx.problem(A, B, T, alpha=1.5), mfx.problem(T, Cm, D, alpha=1.0, beta=1.2)],
+                              deps=[(1, 0)], prec=prec)
+                else:
+                    h.gemm(A, B, T, alpha=1.5, prec=prec)
+                    h.gemm(T, Cm, D, alpha=1.0, beta=1.2, prec=prec)
+
+            def mm3():
+                if grouped:
+                    h.grouped([mfx.problem(A, B, E), mfx.problem(Cm, D0, F), mfx.problem(E, F, G)],
+                              deps=[(2, 0), (2, 1)], prec=prec)
+                else:
+                    h.gemm(A, B, E, prec=prec)
+                    h.gemm(Cm, D0, F, prec=prec)
+                    h.gemm(E, F, G, prec=prec)
+            tag = prec.replace("-", "") + ("_grouped" if grouped else "")
+            ms = _time_ms(torch, mm2, reps)  # D accumulates over the timed calls; re-run once for the error
+            D.copy_(D0)
+            mm2()
+            rec(f"2mm_4096_{tag}", 4.0 * n ** 3, ms, h.rel_frobenius(D, R2))
+            ms = _time_ms(torch, mm3, reps)
+            rec(f"3mm_4096_{tag}", 6.0 * n ** 3, ms, h.rel_frobenius(G, G64))
+    del A, B, Cm, D0, T, D, E, F, G, R2, G64
+    torch.cuda.empty_cache()
+
+    # syrk / syr2k 8192, lower triangle, C symmetric seeded
+    n = 8192
+    A, B, X = seeded((n, n), 0), seeded((n, n), 1), seeded((n, n), 2)
+    C0 = ((X + X.T) * 0.5).contiguous()
+    del X
+    A64, B64 = A.double(), B.double()
+    R1 = C0.double()
+    h.gemm_f64(A64, A64, R1, alpha=1.5, beta=1.2, transB=True)
+    R2 = C0.double()
+    h.gemm_f64(A64, B64, R2, alpha=1.5, beta=1.2, transB=True)
+    h.gemm_f64(B64, A64, R2, alpha=1.5, beta=1.0, transB=True)
+    del A64, B64
+    torch.cuda.empty_cache()
+    C = torch.empty_like(C0)
+    for prec in ("fp32", "split-bf16", "bf16"):
+        tag = prec.replace("-", "")
+        ms = _time_ms(torch, lambda: h.syrk(A, C, alpha=1.5, beta=1.2, prec=prec, uplo=mfx.LOWER, mirror=False), reps)
+        C.copy_(C0)
+        h.syrk(A, C, alpha=1.5, beta=1.2, prec=prec, uplo=mfx.LOWER, mirror=False)
+        rec(f"syrk_8192_lower_{tag}", float(n) * (n + 1) * n, ms, h.rel_frobenius(C, R1, uplo=mfx.LOWER))
+        ms = _time_ms(torch, lambda: h.syr2k(A, B, C, alpha=1.5, beta=1.2, prec=prec, uplo=mfx.LOWER, mirror=False),
+                      reps)
+        C.copy_(C0)
+        h.syr2k(A, B, C, alpha=1.5, beta=1.2, prec=prec, uplo=mfx.LOWER, mirror=False)
+        rec(f"syr2k_8192_lower_{tag}", 2.0 * n * (n + 1) * n, ms, h.rel_frobenius(C, R2, uplo=mfx.LOWER))
+    del A, B, C, C0, R1, R2
+    torch.cuda.empty_cache()
+
+    # matmulchain: ((M0 M1)(M2 M3))((M4 M5)(M6 M7)), 2048^2, one grouped launch
+    n = 2048
+    mats = [seeded((n, n), 10000 + j) for j in range(8)]
+    scratch = [torch.empty((n, n), device=dev) for _ in range(6)]
+    res = torch.empty((n, n), device=dev)
+    level = [m.double() for m in mats]
+    while len(level) > 1:
+        level = [level[i] @ level[i + 1] for i in range(0, len(level), 2)]
+    for prec in ("fp32", "split-bf16"):
+        ms = _time_ms(torch, lambda: h.chain(mats, scratch, res, prec=prec), reps)
+        rec(f"matmulchain_8x2048_{prec.replace('-', '')}", 7 * 2.0 * n ** 3, ms, h.rel_frobenius(res, level[0]))
+    del mats, scratch, res, level
+    torch.cuda.empty_cache()
+
+    # FP8 block-scaled 16384, sign x lognormal(0, 2), error on one 2048^2 block
+    n, blk = 16384, 2048
+    A, B = torch.empty((n, n), device=dev), torch.empty((n, n), device=dev)
+    h.fill_lognormal(A, 7, 0, 0.0, 2.0)
+    h.fill_lognormal(B, 7, 1, 0.0, 2.0)
+    r0, c0 = 5 * 1024, 11 * 1024
+    R = A[r0:r0 + blk].double() @ B[:, c0:c0 + blk].double()
+    C = torch.empty((n, n), device=dev)
+    for prec in ("fp8-block", "fp8-block2"):
+        ms = _time_ms(torch, lambda: h.gemm(A, B, C, prec=prec), reps)
+        err = h.rel_frobenius(C[r0:r0 + blk, c0:c0 + blk], R)
+        rec(f"gemm_16384_{prec.replace('-', '')}", 2.0 * n ** 3, ms, err)
+    del A, B, C, R
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(n, target_s):
     """OpenMP restatement of polybench/gemm.cpp:82-91 on the host cores, timed
     on rows [0, R) of the same N x N problem (R sized to ~target_s)."""
@@ -234,6 +376,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-grouped", action="store_true")
+    ap.add_argument("--all-configs", action="store_true",
+                    help="also measure the other workloads at full size (2mm/3mm 4096, syrk/syr2k 8192, chain, "
+                         "FP8 16384; about a minute more)")
     ap.add_argument("--profile-pass", action="store_true",
                     help="short pass for counter collection: one split-BF16, FP32, single-BF16 and FP8-block gemm "
                          "at --n, plus a 1 GiB device copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters")
@@ -353,6 +498,8 @@ def main():
     cal["fp32_kernel_frac_of_measured_fp32_random"] = round(fp32_kernel_tflops / cal["mfma_fp32_random_tflops"], 4)
     out["measured_ceilings"] = cal
     log(f"measured ceilings: {cal}")
+    if args.all_configs:
+        out["configs"] = measure_configs(torch, mfx, h)
     if not args.no_grouped:
         g = grouped_workload(torch, mfx, h, max(3, args.steps), max(1, args.warmup))
         g["frac_of_large_tile"] = round(g["tflops"] / value, 3)
