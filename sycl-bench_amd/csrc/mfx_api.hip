@@ -143,28 +143,75 @@ void replace_tuned(const std::map<std::string, int>& t) {
   g_tuned = t;
 }
 
-// Heuristic used when nothing is tuned: the largest tile (more waves first
-// among equal tiles) that still gives every CU at least one tile, else the
-// smallest tile.
-int heuristic_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k) {
+// Per-flop rate of each tile configuration relative to the best one, on
+// large problems (the best configuration's time over this one's, measured on
+// MI355X at M = N = K = 8192).
+double config_rate(mfx_precision p, int c) {
+  static const double f32[] = {0.886, 0.868, 0.817, 0.859, 1.0, 0.97, 0.913};
+  static const double split[] = {0.894, 0.846, 0.645, 0.70, 1.0, 0.80, 0.854};
+  static const double bf16[] = {0.844, 0.730, 0.508, 0.662, 1.0, 0.724, 0.814};
+  static const double fp8[] = {0.866, 0.69, 1.0};
+  const double* t = p == MFX_PREC_FP32 ? f32 : p == MFX_PREC_SPLIT_BF16 ? split : p == MFX_PREC_BF16 ? bf16 : fp8;
+  const int n = p == MFX_PREC_FP32 || p == MFX_PREC_SPLIT_BF16 || p == MFX_PREC_BF16 ? 7 : 3;
+  return c >= 0 && c < n ? t[c] : 0.5;
+}
+
+// Heuristic used when nothing is tuned: the configuration with the least
+// estimated time, counted in rounds of tiles over the CUs (a round costs the
+// tile's area over its rate), so a last round that leaves most CUs idle is
+// paid for.  `tiles(bm, bn)` gives the tile count of the schedule (the full
+// grid, or a triangle).  Ties go to the larger tile, then more waves.
+// Exception for the power-limited BF16 / FP8 MFMAs: when the largest tile
+// fills between half and all of the CUs in one round, it is taken (the clock
+// rises as CUs idle, which the round count does not see; measured on MI355X,
+// e.g. a 4096 split-BF16 syrk: 136 large tiles beat 528 small ones by 10 %).
+template <class Tiles>
+int least_time_config(Handle* h, mfx_precision p, bool square_only, Tiles&& tiles) {
   const int cus = h->cu_count > 0 ? h->cu_count : 256;
-  struct Cand {
-    int id, bm, bn, waves;
-  };
-  std::vector<Cand> cands;
+  if(p != MFX_PREC_FP32) {
+    int big = -1, big_w = 0;
+    int64_t big_area = 0;
+    for(int c = 0; c < config_count(p); ++c) {
+      int bm, bn, bk, w;
+      if(!config_info(p, c, &bm, &bn, &bk, &w) || (square_only && bm != bn)) continue;
+      const int64_t area = (int64_t)bm * bn;
+      if(area > big_area || (area == big_area && w > big_w)) {
+        big = c;
+        big_area = area;
+        big_w = w;
+      }
+    }
+    if(big >= 0) {
+      int bm, bn, bk, w;
+      config_info(p, big, &bm, &bn, &bk, &w);
+      const int64_t t = tiles(bm, bn);
+      if(2 * t >= cus && t <= cus) return big;
+    }
+  }
+  int best = -1;
+  double best_t = 0.0;
+  int64_t best_area = 0;
+  int best_waves = 0;
   for(int c = 0; c < config_count(p); ++c) {
     int bm, bn, bk, w;
-    if(config_info(p, c, &bm, &bn, &bk, &w)) cands.push_back({c, bm, bn, w});
+    if(!config_info(p, c, &bm, &bn, &bk, &w) || (square_only && bm != bn)) continue;
+    const int64_t area = (int64_t)bm * bn;
+    const double t = (double)ceil_div(tiles(bm, bn), cus) * (double)area / config_rate(p, c);
+    const bool better = best < 0 || t < best_t * (1.0 - 1e-9) ||
+                        (t <= best_t * (1.0 + 1e-9) && (area > best_area || (area == best_area && w > best_waves)));
+    if(better) {
+      best = c;
+      best_t = t;
+      best_area = area;
+      best_waves = w;
+    }
   }
-  if(cands.empty()) return 0;
-  std::stable_sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) {
-    const int64_t ax = (int64_t)x.bm * x.bn, ay = (int64_t)y.bm * y.bn;
-    return ax != ay ? ax > ay : x.waves > y.waves;
-  });
-  for(const Cand& c : cands)
-    if(ceil_div(m, c.bm) * ceil_div(n, c.bn) >= cus) return c.id;
+  return best < 0 ? 0 : best;
+}
+
+int heuristic_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k) {
   (void)k;
-  return cands.back().id;
+  return least_time_config(h, p, false, [&](int bm, int bn) { return ceil_div(m, bm) * ceil_div(n, bn); });
 }
 
 int choose_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k) {
@@ -184,12 +231,24 @@ int tuned_lookup(const std::string& key) {
   return it == g_tuned.end() ? -1 : it->second;
 }
 
-// Square-tile choice for the triangular schedules.
+// Square-tile choice for the triangular schedules: a forced or tuned square
+// configuration, else the least estimated time over the T(T+1)/2 triangle
+// tiles.
 int choose_square_config(Handle* h, mfx_precision p, int64_t n, int64_t k) {
-  int c = choose_config(h, p, n, n, k);
+  int c = -1;
+  if(h->forced_config[p] >= 0) {
+    c = h->forced_config[p];
+  } else {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    auto it = g_tuned.find(tune_key(p, n, n, k));
+    if(it != g_tuned.end()) c = it->second;
+  }
   int bm, bn, bk, w;
-  if(config_info(p, c, &bm, &bn, &bk, &w) && bm == bn) return c;
-  return 0;
+  if(c >= 0) return config_info(p, c, &bm, &bn, &bk, &w) && bm == bn ? c : 0;
+  return least_time_config(h, p, true, [&](int b, int) {
+    const int64_t t = ceil_div(n, b);
+    return t * (t + 1) / 2;
+  });
 }
 
 mfx_status dispatch_dense(Handle* h, mfx_precision prec, DenseOp& op, int config, hipStream_t s) {
