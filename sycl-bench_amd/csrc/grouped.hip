@@ -10,6 +10,8 @@
 
 namespace mfx {
 
+double config_rate(mfx_precision p, int c);
+
 std::string tune_key(mfx_precision p, int64_t m, int64_t n, int64_t k);
 int tuned_lookup(const std::string& key);
 
@@ -208,6 +210,7 @@ struct GroupedPlan {
   // pre-pass jobs [seg_jobs[l], seg_jobs[l+1]) in seg_blocks[l] blocks.
   bool per_level = false;
   std::vector<int> seg_items, seg_jobs, seg_blocks;
+  std::vector<int> seg_big, seg_small;  // tile pair of each segment's kernel
   const PackJob* dev_jobs = nullptr;          // the job table in device memory
   size_t total_bytes = 0;
   int policy = 0, big = 0, small = 0;  // tile policy and its configurations
@@ -269,8 +272,8 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
   const GroupedPolicy& pol = kPolicies[pl.policy];
   pl.big = grouped_config(prec, pol.big);
   pl.small = grouped_config(prec, pol.small);
-  const std::vector<int> cfg =
-      assign_configs(pol, prec, problems, count, level, h->cu_count > 0 ? h->cu_count : 256);
+  const int cus = h->cu_count > 0 ? h->cu_count : 256;
+  std::vector<int> cfg = assign_configs(pol, prec, problems, count, level, cus);
   const bool packed = prec != MFX_PREC_FP32;
   const int nhl = prec == MFX_PREC_SPLIT_BF16 ? 2 : 1;
   size_t packed_bytes = 0;
@@ -279,6 +282,41 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
   // One launch per level: kernel boundaries order the levels, so no problem
   // waits or publishes in-kernel and every operand is packed by a pre-pass.
   pl.per_level = pol.per_level && max_level > 0;
+  // With a launch per level, a later level whose tiles leave CUs idle (the
+  // last products of a chain) may take a smaller tile of the same workgroup
+  // size for all its problems, in a kernel of its own tile pair, when that
+  // is the least estimated time in rounds over the CUs.
+  std::vector<int> level_cfg(max_level + 1, -1);
+  if(pl.per_level) {
+    const int threads = cfg_threads(pl.big);
+    for(int lv = 1; lv <= max_level; ++lv) {
+      auto level_time = [&](int c) {
+        int64_t tiles = 0;
+        for(int i = 0; i < count; ++i)
+          if(level[i] == lv && problems[i].m > 0 && problems[i].n > 0)
+            tiles += ceil_div(problems[i].m, cfg_bm(c >= 0 ? c : cfg[i])) * ceil_div(problems[i].n, cfg_bn(c >= 0 ? c : cfg[i]));
+        return tiles;
+      };
+      if(level_time(-1) >= cus) continue;
+      double best_t = -1.0;
+      int best = -1;
+      for(int c = 0; c < 7; ++c) {  // tile configurations 0..6
+        if(grouped_config(prec, c) != c || cfg_threads(c) != threads) continue;
+        const double t = (double)ceil_div(level_time(c), cus) * cfg_bm(c) * cfg_bn(c) / config_rate(prec, c);
+        if(best < 0 || t < best_t) {
+          best_t = t;
+          best = c;
+        }
+      }
+      bool uniform = true;
+      for(int i = 0; i < count; ++i) uniform = uniform && (level[i] != lv || cfg[i] == best);
+      if(best >= 0 && !uniform) {
+        level_cfg[lv] = best;
+        for(int i = 0; i < count; ++i)
+          if(level[i] == lv) cfg[i] = best;
+      }
+    }
+  }
 
   for(int i = 0; i < count; ++i) {
     const auto& q = problems[i];
@@ -359,11 +397,15 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
     if(pl.per_level) {
       pl.seg_items.push_back((int)pl.items.size());
       pl.seg_jobs.push_back((int)pl.job_list.size());
+      pl.seg_big.push_back(level_cfg[lv] >= 0 ? level_cfg[lv] : pl.big);
+      pl.seg_small.push_back(level_cfg[lv] >= 0 ? level_cfg[lv] : pl.small);
     }
   }
   if(!pl.per_level) {
     pl.seg_items.push_back((int)pl.items.size());
     pl.seg_jobs.push_back((int)pl.job_list.size());
+    pl.seg_big.push_back(pl.big);
+    pl.seg_small.push_back(pl.small);
   } else {
     for(auto& P : pl.problems) P.ndeps = P.consumed = 0;  // ordered by the launches
   }
@@ -470,8 +512,9 @@ static mfx_status launch_grouped_kernels(Handle* h, const GroupedPlan& pl, const
     gl.nitems = pl.seg_items[l + 1] - pl.seg_items[l];
     gl.head = g.head + l;
     if(gl.nitems == 0) continue;
-    if(pl.prec == MFX_PREC_FP32) MFX_CHECK(launch_grouped_f32(h, pl.big, pl.small, pl.all_vec, gl, grid, s));
-    else MFX_CHECK(launch_grouped_bf16(h, nhl, pl.big, pl.small, pl.all_vec, gl, grid, s));
+    const int big = pl.seg_big[l], small = pl.seg_small[l];
+    if(pl.prec == MFX_PREC_FP32) MFX_CHECK(launch_grouped_f32(h, big, small, pl.all_vec, gl, grid, s));
+    else MFX_CHECK(launch_grouped_bf16(h, nhl, big, small, pl.all_vec, gl, grid, s));
   }
   return MFX_SUCCESS;
 }
