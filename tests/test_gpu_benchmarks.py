@@ -143,3 +143,24 @@ def test_edge_sizes(exe):
         assert res["Verification"] == "PASS", n
     r, res = bench(exe, "--size=33", "--num-runs=1", "--init=uniform", "--precision=split-bf16")
     assert r.returncode == 0 and res["Verification"] == "PASS", r.stderr
+
+
+@pytest.mark.parametrize("prec", ["fp8-block", "fp8-block2"])
+@pytest.mark.parametrize("exe", ["gemm", "2mm", "3mm", "matmulchain"])
+def test_fp8_variants(exe, prec):
+    r, res = bench(exe, "--size=512", "--num-runs=1", "--init=uniform", f"--precision={prec}", "--metrics=extended")
+    assert r.returncode == 0, r.stderr
+    assert res["Verification"] == "PASS", res
+    # the programs' FP8 tolerances: one term 0.25, two terms 0.05 (per chained product: x sqrt(stages))
+    assert float(res["rel-frobenius-error"]) < {"fp8-block": 0.25, "fp8-block2": 0.05}[prec] * 3 ** 0.5
+    assert res["_names"][0].endswith("_" + prec.replace("-", "") + "_uniform")
+
+
+def test_fp8_lognormal_gemm_and_grouped_refusal():
+    r, res = bench("gemm", "--size=1024", "--num-runs=1", "--init=lognormal", "--precision=fp8-block2",
+                   "--metrics=extended")
+    assert r.returncode == 0 and res["Verification"] == "PASS", r.stderr
+    # FP8 has no grouped kernel: the grouped form says so instead of running
+    r, _ = bench("3mm", "--size=256", "--num-runs=1", "--precision=fp8-block", "--grouped")
+    assert r.returncode != 0
+    assert "grouped" in (r.stderr + r.stdout).lower()
