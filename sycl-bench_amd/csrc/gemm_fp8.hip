@@ -83,8 +83,13 @@ __device__ __forceinline__ float e4m3_byte(uint32_t w) {
 // sources put consecutive threads along k (64-byte vector loads per
 // thread); row-contiguous ones put them along the rows (coalesced across
 // lanes).
+// Row pitch (bytes) of the LDS code tile: 128 codes + 16, so 16-byte reads
+// of one column by consecutive rows spread over the banks.
+constexpr int QPITCH = 144;
+
 __global__ __launch_bounds__(256) void quantize_fp8_kernel(const QuantParams p) {
   __shared__ float sh[4];
+  __shared__ __align__(16) unsigned char qtile[2][128 * QPITCH];
   // Neighbouring workgroups take neighbouring blocks along the source's
   // contiguous dimension (k if k-major, rows if not), so they read
   // neighbouring segments of the same memory rows.
@@ -162,16 +167,28 @@ __global__ __launch_bounds__(256) void quantize_fp8_kernel(const QuantParams p) 
       for(int j = 0; j < 4; ++j)
         w2[j] = e4m3_pack4(v[c][4 * j] * inv2, v[c][4 * j + 1] * inv2, v[c][4 * j + 2] * inv2, v[c][4 * j + 3] * inv2);
     if(p.q1) {
-      // fragment-order position of (r, k0..k0+15)
-      const int kb64s = p.kb128s * 2;
-      const int blk = r / 32, kb64 = k0 / 64, w64 = k0 % 64;
-      const int lane = (r % 32) + 32 * (w64 / 32), half = (w64 % 32) / 16;
-      const int64_t at = (((int64_t)blk * kb64s + kb64) * 2 + half) * 64 + lane;
-      p.q1[at] = (u32x4){w1[c][0], w1[c][1], w1[c][2], w1[c][3]};
-      if(p.two_term) p.q2[at] = (u32x4){w2[0], w2[1], w2[2], w2[3]};
+      // codes of (rr, kk*16 .. kk*16+15) into the block's LDS tile
+      *reinterpret_cast<u32x4*>(&qtile[0][rr * QPITCH + kk * 16]) = (u32x4){w1[c][0], w1[c][1], w1[c][2], w1[c][3]};
+      if(p.two_term) *reinterpret_cast<u32x4*>(&qtile[1][rr * QPITCH + kk * 16]) = (u32x4){w2[0], w2[1], w2[2], w2[3]};
     } else if(r < p.rows) {
       for(int e = 0; e < 16 && k0 + e < p.K; ++e)
         p.plain[(int64_t)r * p.plain_ld + k0 + e] = (uint8_t)(w1[c][e / 4] >> (8 * (e % 4)));
+    }
+  }
+  if(p.q1) {
+    // The block's 1024 fragment-order chunks (block = 128 here), consecutive
+    // threads on consecutive chunks: chunk (blk, kb64, half, lane) holds the
+    // codes of row blk*32 + lane%32, k = kb64*64 + 32*(lane/32) + 16*half + 0..15.
+    __syncthreads();
+    const int kb64s = p.kb128s * 2;
+#pragma unroll
+    for(int c = 0; c < 4; ++c) {
+      const int ch = threadIdx.x + c * 256;
+      const int lane = ch & 63, half = (ch >> 6) & 1, kb64l = (ch >> 7) & 1, blkl = ch >> 8;
+      const int row = blkl * 32 + (lane & 31), col = kb64l * 64 + 32 * (lane >> 5) + 16 * half;
+      const int64_t at = ((((int64_t)(rb * 4 + blkl)) * kb64s + kb * 2 + kb64l) * 2 + half) * 64 + lane;
+      p.q1[at] = *reinterpret_cast<const u32x4*>(&qtile[0][row * QPITCH + col]);
+      if(p.two_term) p.q2[at] = *reinterpret_cast<const u32x4*>(&qtile[1][row * QPITCH + col]);
     }
   }
 }
