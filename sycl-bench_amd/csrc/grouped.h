@@ -16,13 +16,15 @@
 //     panel); TILE (FP32, which reads the operands directly): the producers.
 // Pack items are small (about 1-2 us each) so a level's packing spreads over
 // all workgroups at HBM speed instead of one workgroup per panel.
-// Completion is published by fence + barrier + atomic increment; waits are
-// acquire loads at agent scope, so producer results written on another XCD
-// are visible.  Deadlock freedom: a workgroup only waits for items that
-// precede its own in the queue, and each of those was taken by a workgroup
-// that is running and waits only for items before it.  A wait that exceeds
-// its spin limit sets an error flag and skips the item; every workgroup still
-// drains the queue and exits.
+// Completion is published by fence + barrier + atomic increment; a wait is
+// polled by one thread with relaxed loads and ends in an agent-scope
+// acquire, so producer results written on another XCD are visible.
+// Deadlock freedom: a workgroup only waits for items that precede its own in
+// the queue, and each of those was taken by a workgroup that is running and
+// waits only for items before it.  A wait that exceeds
+// its spin limit sets an error flag and skips the item; every other wait
+// gives up at its next poll once the flag is set, so every workgroup drains
+// the queue and exits promptly.
 //
 // Tile shapes: a launch runs one tile policy, a large and a small tile
 // configuration; each problem takes the large tile unless that pads its
@@ -122,27 +124,48 @@ __host__ __device__ inline int cfg_threads(int config) { return config >= 4 ? 51
 // (performed at the coherent level, no invalidation) and acquire once when
 // the condition holds, and a signal writes back once per workgroup.
 
-// Bounded wait for *ctr >= target (all threads of the workgroup).
-__device__ __forceinline__ bool wait_at_least(int* ctr, int target, const GroupedArgs& g) {
+// Bounded poll for *ctr >= target, by one thread.  It gives up as soon as
+// any workgroup has set the error flag (a wait elsewhere timed out, so the
+// item this one waits for may never be published), or after spin_limit
+// polls, when it sets the flag itself.  So one expired wait ends every other
+// wait of the launch within a poll instead of each spinning its own limit.
+__device__ __forceinline__ bool poll_at_least(int* ctr, int target, const GroupedArgs& g) {
   int spins = 0;
   while(__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    if(__hip_atomic_load(g.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
     if(++spins > g.spin_limit) {
       __hip_atomic_store(g.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
     }
     __builtin_amdgcn_s_sleep(8);
   }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return true;
 }
 
-__device__ __forceinline__ bool wait_producers(const GProblem& P, const GroupedArgs& g) {
-  bool ok = true;
+// Polls every producer of P in turn, stopping at the first that fails.
+__device__ __forceinline__ bool poll_producers(const GProblem& P, const GroupedArgs& g) {
   for(int d = 0; d < P.ndeps; ++d) {
     const GProblem& Q = g.problems[P.deps[d]];
-    ok = wait_at_least(&g.done[P.deps[d]], Q.tiles_m * Q.tiles_n, g) && ok;
+    if(!poll_at_least(&g.done[P.deps[d]], Q.tiles_m * Q.tiles_n, g)) return false;
   }
-  return ok;
+  return true;
+}
+
+// Workgroup-uniform wait: thread 0 runs `poll` (the other waves do not add
+// polling traffic), the answer is shared through `flag` in LDS, so the whole
+// workgroup either runs the item or skips it -- never half of it, which
+// would leave the tile's barriers unmatched.  On success thread 0's
+// agent-scope acquire makes the producers' results visible to it, and the
+// barrier orders every other thread's later loads after that acquire.
+template <class Poll>
+__device__ __forceinline__ bool wg_wait(int* flag, Poll&& poll) {
+  if(threadIdx.x == 0) {
+    const bool ok = poll();
+    if(ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    *flag = ok ? 1 : 0;
+  }
+  __syncthreads();
+  return *flag != 0;
 }
 
 // Publishes the workgroup's global writes, then bumps *ctr once: every thread

@@ -85,7 +85,7 @@ __global__ __launch_bounds__(geo_threads<BIG>()) void grouped_bf16x_kernel(const
   static_assert(geo_threads<SMALL>() == NT, "both tiles of a policy use the same workgroup");
   constexpr int S0 = bf16_smem_chunks<NHL, BIG>(), S1 = bf16_smem_chunks<NHL, SMALL>();
   __shared__ u32x4 lds[S0 > S1 ? S0 : S1];
-  __shared__ int slot;
+  __shared__ int slot, ready;
   for(;;) {
     const int it = take_item(g.head, &slot);
     if(it >= g.nitems) break;
@@ -94,20 +94,26 @@ __global__ __launch_bounds__(geo_threads<BIG>()) void grouped_bf16x_kernel(const
     const int bm = cfg_bm(P.config), bn = cfg_bn(P.config);
     if(item.type != ITEM_TILE) {
       const bool is_a = item.type == ITEM_PACK_A;
-      if(!wait_producers(P, g)) continue;
+      if(P.ndeps > 0 && !wg_wait(&ready, [&] { return poll_producers(P, g); })) continue;
       pack_item<NHL, NT>(P, is_a, item.x, item.y);
       const int panel = item.x * 32 / (is_a ? bm : bn);
       signal(&g.pack_ctr[(is_a ? P.pack_a_ctr : P.pack_b_ctr) + panel]);
       continue;
     }
     const int nblk_a = (P.M + 31) / 32, nblk_b = (P.N + 31) / 32;
-    if(!P.prepacked) {
+    if(!P.prepacked && P.pack_slices > 0) {
       // pack items of a panel: its 32-row blocks that meet the matrix, each in pack_slices k slices
       const int need_a = (min((item.x + 1) * (bm / 32), nblk_a) - item.x * (bm / 32)) * P.pack_slices;
       const int need_b = (min((item.y + 1) * (bn / 32), nblk_b) - item.y * (bn / 32)) * P.pack_slices;
-      const bool ready = wait_at_least(&g.pack_ctr[P.pack_a_ctr + item.x], need_a, g) &&
-                         wait_at_least(&g.pack_ctr[P.pack_b_ctr + item.y], need_b, g);
-      if(!ready) continue;
+      if(!wg_wait(&ready, [&] {
+           return poll_at_least(&g.pack_ctr[P.pack_a_ctr + item.x], need_a, g) &&
+                  poll_at_least(&g.pack_ctr[P.pack_b_ctr + item.y], need_b, g);
+         }))
+        continue;
+    } else if(!P.prepacked && P.ndeps > 0) {
+      // K == 0: no pack items, but the tile still rewrites C (beta * C), so
+      // it waits for the producers like any other consumer
+      if(!wg_wait(&ready, [&] { return poll_producers(P, g); })) continue;
     }
     BfTileArgs t;
     t.PA = P.packed_a;

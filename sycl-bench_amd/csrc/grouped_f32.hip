@@ -27,13 +27,13 @@ __global__ __launch_bounds__(geo_threads<BIG>()) void grouped_f32_kernel(const G
   static_assert(geo_threads<SMALL>() == geo_threads<BIG>(), "both tiles of a policy use the same workgroup");
   constexpr int S0 = f32_smem_floats<BIG>(), S1 = f32_smem_floats<SMALL>();
   __shared__ float smem[S0 > S1 ? S0 : S1];
-  __shared__ int slot;
+  __shared__ int slot, ready;
   for(;;) {
     const int it = take_item(g.head, &slot);
     if(it >= g.nitems) break;
     const GItem item = g.items[it];
     const GProblem& P = g.problems[item.problem];
-    if(!wait_producers(P, g)) continue;
+    if(P.ndeps > 0 && !wg_wait(&ready, [&] { return poll_producers(P, g); })) continue;
     DenseOp op{};
     op.M = P.M;
     op.N = P.N;
