@@ -62,6 +62,16 @@ mfx_status mfx_destroy(mfx_handle handle);
 mfx_status mfx_reserve_workspace(mfx_handle handle, size_t bytes);
 /* Workspace bytes a gemm of this shape and precision needs (0 for FP32). */
 size_t mfx_gemm_workspace_size(mfx_precision prec, int64_t m, int64_t n, int64_t k);
+/*
+ * Host-only pre-flight check (no handle, no device): the status mfx_gemm
+ * returns for these enums, shapes and leading dimensions before it touches
+ * any memory -- MFX_ERROR_INVALID_VALUE, MFX_ERROR_NOT_SUPPORTED (K beyond
+ * 2^25 for SPLIT_BF16 or 2^26 for BF16, the packed paths' 32-bit panel
+ * offsets) or MFX_SUCCESS.  Operand pointers are not checked.
+ * No reference counterpart (SYCL-Bench validates nothing before submitting).
+ */
+mfx_status mfx_gemm_check(mfx_precision prec, mfx_op transA, mfx_op transB, int64_t m, int64_t n, int64_t k,
+    int64_t lda, int64_t ldb, int64_t ldc);
 const char* mfx_status_string(mfx_status status);
 /* Text of the last error seen on the calling thread (HIP error names included). */
 const char* mfx_last_error(void);

@@ -218,7 +218,8 @@ struct GroupedPlan {
   double flops = 0.0;
 };
 
-static mfx_status validate(const mfx_gemm_problem* problems, int count, const mfx_dep* deps, int ndeps) {
+static mfx_status validate(mfx_precision prec, const mfx_gemm_problem* problems, int count, const mfx_dep* deps,
+    int ndeps) {
   if(count < 0 || ndeps < 0) return invalid("negative problem or dependency count");
   if(count > 0 && !problems) return invalid("null problem list");
   if(ndeps > 0 && !deps) return invalid("null dependency list");
@@ -231,6 +232,8 @@ static mfx_status validate(const mfx_gemm_problem* problems, int count, const mf
     const int64_t a_rows = q.transA ? q.k : q.m, a_cols = q.transA ? q.m : q.k;
     const int64_t b_rows = q.transB ? q.n : q.k, b_cols = q.transB ? q.k : q.n;
     if(q.m > 0 && q.n > 0) {
+      if(prec != MFX_PREC_FP32 && !bf16_k_supported(q.k, prec == MFX_PREC_SPLIT_BF16 ? 2 : 1))
+        return unsupported("problem " + std::to_string(i) + ": K is beyond the packed BF16 path's limit");
       if(!q.C || q.ldc < std::max<int64_t>(1, q.n)) return invalid("problem " + std::to_string(i) + ": bad C");
       if(q.k > 0 && (!q.A || !q.B)) return invalid("problem " + std::to_string(i) + ": null operand");
       if(q.lda < std::max<int64_t>(1, a_cols) || q.ldb < std::max<int64_t>(1, b_cols))
@@ -252,7 +255,7 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
     const mfx_dep* deps, int ndeps, GroupedPlan& pl) {
   if(prec != MFX_PREC_FP32 && prec != MFX_PREC_SPLIT_BF16 && prec != MFX_PREC_BF16)
     return unsupported("grouped launch supports FP32, SPLIT_BF16 and BF16");
-  MFX_CHECK(validate(problems, count, deps, ndeps));
+  MFX_CHECK(validate(prec, problems, count, deps, ndeps));
   pl.prec = prec;
   pl.problems.resize(count);
   std::vector<int> level(count, 0);

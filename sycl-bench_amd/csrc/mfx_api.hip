@@ -4,7 +4,7 @@
 #include <cstdio>
 #include <map>
 
-#include "mfx_internal.h"
+#include "tile_bf16.h"
 
 namespace mfx {
 
@@ -273,6 +273,27 @@ mfx_status check_matrix(const char* name, const void* p, int64_t rows, int64_t c
   return MFX_SUCCESS;
 }
 
+// K limit of the packed precisions (32-bit panel offsets in the BF16 tile).
+mfx_status check_k(mfx_precision prec, int64_t k) {
+  if((prec == MFX_PREC_SPLIT_BF16 || prec == MFX_PREC_BF16) && !bf16_k_supported(k, prec == MFX_PREC_SPLIT_BF16 ? 2 : 1))
+    return unsupported("K = " + std::to_string(k) + " is beyond the packed BF16 path's limit (2^25 split, 2^26 single)");
+  return MFX_SUCCESS;
+}
+
+// Argument checks of mfx_gemm; pointers are checked only when check_ptrs.
+mfx_status check_gemm_args(mfx_precision prec, mfx_op transA, mfx_op transB, int64_t m, int64_t n, int64_t k,
+    const void* A, int64_t lda, const void* B, int64_t ldb, const void* C, int64_t ldc) {
+  if(!valid_prec(prec)) return invalid("unknown precision");
+  if((transA != MFX_OP_N && transA != MFX_OP_T) || (transB != MFX_OP_N && transB != MFX_OP_T))
+    return invalid("transA / transB must be MFX_OP_N or MFX_OP_T");
+  if(m < 0 || n < 0 || k < 0) return invalid("negative dimension");
+  MFX_CHECK(transA == MFX_OP_N ? check_matrix("A", A, m, k, lda) : check_matrix("A", A, k, m, lda));
+  MFX_CHECK(transB == MFX_OP_N ? check_matrix("B", B, k, n, ldb) : check_matrix("B", B, n, k, ldb));
+  MFX_CHECK(check_matrix("C", C, m, n, ldc));
+  if(m == 0 || n == 0) return MFX_SUCCESS;
+  return check_k(prec, k);
+}
+
 // C (n x n) = alpha * A·Aᵀ (syr2k: A·Bᵀ + B·Aᵀ) + beta * C on the
 // triangle schedule.  A is n x k row-major, or with `trans` k x n (so the
 // product is Aᵀ·A, the covariance form).
@@ -286,6 +307,7 @@ mfx_status sym_common(Handle* h, mfx_precision prec, mfx_uplo uplo, int mirror, 
   if(nprod == 2) MFX_CHECK(trans ? check_matrix("B", B, k, n, ldb) : check_matrix("B", B, n, k, ldb));
   MFX_CHECK(check_matrix("C", C, n, n, ldc));
   if(n == 0) return MFX_SUCCESS;
+  MFX_CHECK(check_k(prec, k));
   DenseOp op{};
   op.M = op.N = (int)n;
   op.K = (int)k;
@@ -371,6 +393,13 @@ mfx_status mfx_reserve_workspace(mfx_handle handle, size_t bytes) {
   return workspace(h, bytes, &p);
 }
 
+mfx_status mfx_gemm_check(mfx_precision prec, mfx_op transA, mfx_op transB, int64_t m, int64_t n, int64_t k,
+    int64_t lda, int64_t ldb, int64_t ldc) {
+  // any non-null address stands in for the operands: only shapes are checked
+  static const float stand_in = 0.0f;
+  return check_gemm_args(prec, transA, transB, m, n, k, &stand_in, lda, &stand_in, ldb, &stand_in, ldc);
+}
+
 size_t mfx_gemm_workspace_size(mfx_precision prec, int64_t m, int64_t n, int64_t k) {
   if(!valid_prec(prec) || m <= 0 || n <= 0 || k <= 0) return 0;
   DenseOp op{};
@@ -418,13 +447,7 @@ mfx_status mfx_gemm(mfx_handle handle, mfx_precision prec, mfx_op transA, mfx_op
     int64_t k, float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc,
     mfx_stream stream) {
   HANDLE_OR_FAIL(handle);
-  if(!valid_prec(prec)) return invalid("unknown precision");
-  if((transA != MFX_OP_N && transA != MFX_OP_T) || (transB != MFX_OP_N && transB != MFX_OP_T))
-    return invalid("transA / transB must be MFX_OP_N or MFX_OP_T");
-  if(m < 0 || n < 0 || k < 0) return invalid("negative dimension");
-  MFX_CHECK(transA == MFX_OP_N ? check_matrix("A", A, m, k, lda) : check_matrix("A", A, k, m, lda));
-  MFX_CHECK(transB == MFX_OP_N ? check_matrix("B", B, k, n, ldb) : check_matrix("B", B, n, k, ldb));
-  MFX_CHECK(check_matrix("C", C, m, n, ldc));
+  MFX_CHECK(check_gemm_args(prec, transA, transB, m, n, k, A, lda, B, ldb, C, ldc));
   if(m == 0 || n == 0) return MFX_SUCCESS;
   DenseOp op{};
   op.M = (int)m;

@@ -33,6 +33,16 @@ constexpr int kb_per_stage() {
 }
 __host__ __device__ inline int kb_per_stage_of(int nhl) { return nhl == 2 ? 2 : 4; }
 
+// bf16_tile keeps each thread's chunk offsets within a tile's panel in 32
+// bits: at most (BM / 32) blocks of kbs * NHL * 64 chunks, BM <= 256.  So the
+// packed paths take K only while 8 * kbs * NHL * 64 < 2^31, i.e. K below
+// 2^25 (split BF16) or 2^26 (single BF16).
+inline bool bf16_k_supported(int64_t k, int nhl) {
+  const int64_t stage = 16 * kb_per_stage_of(nhl);
+  const int64_t kbs = (k + stage - 1) / stage * stage / 16;
+  return 8 * kbs * nhl * 64 < ((int64_t)1 << 31);
+}
+
 __device__ __forceinline__ uint32_t pack_bf16_pair(uint16_t a, uint16_t b) {
   return (uint32_t)a | ((uint32_t)b << 16);
 }

@@ -46,6 +46,7 @@ SIGNATURES = {
     "mfx_destroy": (_ST, [_V]),
     "mfx_reserve_workspace": (_ST, [_V, ctypes.c_size_t]),
     "mfx_gemm_workspace_size": (ctypes.c_size_t, [_ST, _I64, _I64, _I64]),
+    "mfx_gemm_check": (_ST, [_ST, _ST, _ST, _I64, _I64, _I64, _I64, _I64, _I64]),
     "mfx_status_string": (ctypes.c_char_p, [_ST]),
     "mfx_last_error": (ctypes.c_char_p, []),
     "mfx_device_name": (_ST, [_V, ctypes.c_char_p, ctypes.c_size_t]),

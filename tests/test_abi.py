@@ -74,3 +74,28 @@ def test_null_handle_is_rejected(mfx):
     st = lib.mfx_gemm(None, 0, 0, 0, 4, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 4, None)
     assert st == 1  # MFX_ERROR_INVALID_VALUE
     assert b"null handle" in lib.mfx_last_error()
+
+
+def test_gemm_check_is_host_only_and_bounds_k(mfx):
+    """The pre-flight check needs no device; the packed BF16 paths refuse K at
+    the limit of their 32-bit panel offsets (8 * kbs * planes * 64 < 2^31)."""
+    lib = mfx.load()
+    N, T = mfx.OP_N, mfx.OP_T
+    assert lib.mfx_gemm_check(mfx.SPLIT_BF16, N, N, 64, 64, 64, 64, 64, 64) == 0
+    assert lib.mfx_gemm_check(mfx.FP32, N, N, 64, 64, 64, 63, 64, 64) == 1  # lda < k
+    assert lib.mfx_gemm_check(mfx.FP32, T, N, 64, 64, 100, 64, 64, 64) == 0  # A is k x m: lda >= m
+    assert lib.mfx_gemm_check(7, N, N, 64, 64, 64, 64, 64, 64) == 1
+    assert lib.mfx_gemm_check(mfx.FP32, N, N, -1, 64, 64, 64, 64, 64) == 1
+    # split BF16: k blocks of 16 padded to the 32-wide stage; 8 * kbs * 2 * 64 < 2^31  <=>  K <= 2^25 - 32
+    k_max = (1 << 25) - 32
+    assert lib.mfx_gemm_check(mfx.SPLIT_BF16, N, T, 32, 32, k_max, k_max, k_max, 32) == 0
+    assert lib.mfx_gemm_check(mfx.SPLIT_BF16, N, T, 32, 32, k_max + 1, k_max + 1, k_max + 1, 32) == 2
+    assert b"limit" in lib.mfx_last_error()
+    # single BF16: 64-wide stage, one plane  <=>  K <= 2^26 - 64
+    k1 = (1 << 26) - 64
+    assert lib.mfx_gemm_check(mfx.BF16, N, T, 32, 32, k1, k1, k1, 32) == 0
+    assert lib.mfx_gemm_check(mfx.BF16, N, T, 32, 32, k1 + 1, k1 + 1, k1 + 1, 32) == 2
+    # FP32 and FP8 index with 64-bit offsets: no such limit
+    assert lib.mfx_gemm_check(mfx.FP32, N, T, 32, 32, 1 << 30, 1 << 30, 1 << 30, 32) == 0
+    # empty outputs need no contraction, so no K limit applies
+    assert lib.mfx_gemm_check(mfx.SPLIT_BF16, N, T, 0, 32, 1 << 30, 1 << 30, 1 << 30, 32) == 0
