@@ -8,15 +8,17 @@
 // scale, upper-triangle correlation + mirror, unit diagonal).
 //
 // Column sums run as a two-pass reduction: a grid of (256-column strip x row
-// slab) blocks writes per-slab partial sums (double) to the workspace, a
-// second kernel adds the slabs of each column in slab order, so results do
-// not depend on scheduling.  The arithmetic of each finished statistic
-// follows the reference's types: the covariance mean divides a float sum by
-// a float constant; the correlation standard deviation divides the float sum
-// of float squares by the double constant, takes a float square root and
-// replaces values <= eps by 1; centring subtracts in float, and the
-// correlation scaling divides in double by sqrt(n_const) and in float by the
-// deviation.
+// slab) blocks writes per-slab partial sums to the workspace, a second kernel
+// adds the slabs of each column in slab order, so results do not depend on
+// scheduling.  Every sum is float, as in the reference, made compensated
+// (Neumaier) so its error does not grow with the row count.  The arithmetic
+// of each finished statistic follows the reference's types: the covariance
+// mean divides a float sum by a float constant; the correlation standard
+// deviation divides the float sum of float squares by the double constant
+// FLOAT_N, takes a float square root and replaces values <= eps by 1;
+// centring subtracts in float, and the correlation scaling divides by
+// sqrt(FLOAT_N) in double and by the deviation in float.  Those two scalar
+// divisions are the only double operations, and they are the reference's own.
 #include <algorithm>
 #include <cmath>
 
@@ -31,15 +33,28 @@ mfx_status check_matrix(const char* name, const void* p, int64_t rows, int64_t c
 
 namespace {
 
+// Compensated (Neumaier) float summation: `sum` plus the running rounding
+// error `comp`, so a long column sum stays as accurate as a float can hold
+// without widening the type the reference sums in.
+struct FSum {
+  float sum = 0.0f, comp = 0.0f;
+  __device__ __forceinline__ void add(float x) {
+    const float t = sum + x;
+    comp += fabsf(sum) >= fabsf(x) ? (sum - t) + x : (x - t) + sum;
+    sum = t;
+  }
+  __device__ __forceinline__ float value() const { return sum + comp; }
+};
+
 // Partial column sums of rows [slab * rows_per, ...): mode 0 sums x, mode 1
 // sums (x - mean[j])^2 with the difference and square in float.
 __global__ __launch_bounds__(256) void column_partial_kernel(int mode, int64_t m, int64_t n, const float* data,
-    int64_t ld, const float* mean, int64_t rows_per, double* partial) {
+    int64_t ld, const float* mean, int64_t rows_per, float* partial) {
   const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if(j >= m) return;
   const int64_t r0 = (int64_t)blockIdx.y * rows_per, r1 = min(n, r0 + rows_per);
   const float mu = mode == 1 ? mean[j] : 0.0f;
-  double acc = 0.0;
+  FSum acc;
   int64_t r = r0;
   // four independent loads in flight per thread
   for(; r + 4 <= r1; r += 4) {
@@ -50,9 +65,9 @@ __global__ __launch_bounds__(256) void column_partial_kernel(int mode, int64_t m
     for(int u = 0; u < 4; ++u) {
       if(mode == 1) {
         const float d = x[u] - mu;
-        acc += (double)(d * d);
+        acc.add(d * d);
       } else {
-        acc += (double)x[u];
+        acc.add(x[u]);
       }
     }
   }
@@ -60,31 +75,32 @@ __global__ __launch_bounds__(256) void column_partial_kernel(int mode, int64_t m
     const float x = data[r * ld + j];
     if(mode == 1) {
       const float d = x - mu;
-      acc += (double)(d * d);
+      acc.add(d * d);
     } else {
-      acc += (double)x;
+      acc.add(x);
     }
   }
-  partial[(int64_t)blockIdx.y * m + j] = acc;
+  partial[(int64_t)blockIdx.y * m + j] = acc.value();
 }
 
-// mean[j] = (float)(sum over slabs) / n_const, in float.
-__global__ void finish_mean_kernel(int64_t m, int slabs, const double* partial, float n_const, float* mean) {
+// mean[j] = (sum over slabs) / n_const, in float.
+__global__ void finish_mean_kernel(int64_t m, int slabs, const float* partial, float n_const, float* mean) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(j >= m) return;
-  double s = 0.0;
-  for(int q = 0; q < slabs; ++q) s += partial[(int64_t)q * m + j];
-  mean[j] = (float)s / n_const;
+  FSum s;
+  for(int q = 0; q < slabs; ++q) s.add(partial[(int64_t)q * m + j]);
+  mean[j] = s.value() / n_const;
 }
 
-// stddev[j] = sqrt((float)((float)sum / n_const)) in float, <= eps -> 1.
-__global__ void finish_stddev_kernel(int64_t m, int slabs, const double* partial, double n_const, double eps,
+// stddev[j] = sqrtf(sum / n_const) with the division in double (the
+// reference's FLOAT_N is a double constant), <= eps -> 1.
+__global__ void finish_stddev_kernel(int64_t m, int slabs, const float* partial, double n_const, double eps,
     float* stddev) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(j >= m) return;
-  double s = 0.0;
-  for(int q = 0; q < slabs; ++q) s += partial[(int64_t)q * m + j];
-  float v = (float)((double)(float)s / n_const);
+  FSum s;
+  for(int q = 0; q < slabs; ++q) s.add(partial[(int64_t)q * m + j]);
+  float v = (float)((double)s.value() / n_const);
   v = sqrtf(v);
   stddev[j] = (double)v <= eps ? 1.0f : v;
 }
@@ -119,11 +135,11 @@ int64_t slab_count(int64_t m, int64_t n) {
 }
 
 mfx_status column_stats(Handle* h, int mode, int64_t m, int64_t n, const float* data, int64_t ld, const float* mean,
-    int64_t* slabs_out, double** partial_out, hipStream_t s) {
+    int64_t* slabs_out, float** partial_out, hipStream_t s) {
   const int64_t slabs = slab_count(m, n);
   void* ws = nullptr;
-  MFX_CHECK(workspace(h, (size_t)(slabs * m) * sizeof(double), &ws));
-  double* partial = static_cast<double*>(ws);
+  MFX_CHECK(workspace(h, (size_t)(slabs * m) * sizeof(float), &ws));
+  float* partial = static_cast<float*>(ws);
   const int64_t rows_per = ceil_div(n, slabs);
   MFX_CHECK(timed_launch(h, mode ? "column_sqdev_partial" : "column_sum_partial", s, [&] {
     column_partial_kernel<<<dim3((unsigned)ceil_div(m, 256), (unsigned)slabs), 256, 0, s>>>(mode, m, n, data, ld,
@@ -160,7 +176,7 @@ mfx_status mfx_covariance(mfx_handle handle, mfx_precision prec, int64_t m, int6
   if(!mean) return invalid("null mean");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int64_t slabs = 0;
-  double* partial = nullptr;
+  float* partial = nullptr;
   MFX_CHECK(column_stats(h, 0, m, n, data, ld, nullptr, &slabs, &partial, s));
   MFX_CHECK(timed_launch(h, "column_mean", s, [&] {
     finish_mean_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, s>>>(m, (int)slabs, partial, n_const, mean);
@@ -183,7 +199,7 @@ mfx_status mfx_correlation(mfx_handle handle, mfx_precision prec, int64_t m, int
   if(!mean || !stddev) return invalid("null mean or stddev");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int64_t slabs = 0;
-  double* partial = nullptr;
+  float* partial = nullptr;
   MFX_CHECK(column_stats(h, 0, m, n, data, ld, nullptr, &slabs, &partial, s));
   MFX_CHECK(timed_launch(h, "column_mean", s, [&] {
     finish_mean_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, s>>>(m, (int)slabs, partial, (float)n_const, mean);
