@@ -54,6 +54,34 @@ void cpu_gemm_rows(const float* A, const float* B, float* C, int64_t n, float al
   }
 }
 
+/* The same kernel body on an m x n x k problem (row-major A m x k, B k x n,
+ * C m x n), rows [row0, row1): one problem of the batched uneven workload,
+ * which the reference would run as one gemm submission per problem. */
+void cpu_gemm_mnk_rows(const float* A, const float* B, float* C, int64_t m, int64_t n, int64_t k, float alpha,
+    float beta, int64_t row0, int64_t row1) {
+  (void)m;
+#pragma omp parallel for schedule(static)
+  for(int64_t i = row0; i < row1; ++i) {
+    for(int64_t j0 = 0; j0 < n; j0 += LANES) {
+      const int64_t w = n - j0 < LANES ? n - j0 : LANES;
+      float acc[LANES];
+#pragma omp simd
+      for(int64_t l = 0; l < LANES; ++l) acc[l] = (l < w && beta != 0.0f) ? C[i * n + j0 + l] * beta : 0.0f;
+      for(int64_t kk = 0; kk < k; ++kk) {
+        const float a = alpha * A[i * k + kk];
+        const float* b = B + kk * n + j0;
+        if(w == LANES) {
+#pragma omp simd
+          for(int64_t l = 0; l < LANES; ++l) acc[l] += a * b[l];
+        } else {
+          for(int64_t l = 0; l < w; ++l) acc[l] += a * b[l];
+        }
+      }
+      for(int64_t l = 0; l < w; ++l) C[i * n + j0 + l] = acc[l];
+    }
+  }
+}
+
 /* polybench/syrk.cpp:74-83 over rows [row0, row1):
  * C[i,j] *= beta; for k: C[i,j] += alpha * A[i,k] * A[j,k]. */
 void cpu_syrk_rows(const float* A, float* C, int64_t n, float alpha, float beta, int64_t row0, int64_t row1) {
@@ -82,7 +110,27 @@ void cpu_syr2k_rows(const float* A, const float* B, float* C, int64_t n, float a
   }
 }
 
-/* runtime/matmulchain.cpp:20-28 (FP32 accumulator) over rows [row0, row1): c = a*b. */
+/* runtime/matmulchain.cpp:20-28 (FP32 accumulator) over rows [row0, row1):
+ * c = a*b, c written without being read (the reference's discard_write). */
 void cpu_matmul_rows(const float* A, const float* B, float* C, int64_t n, int64_t row0, int64_t row1) {
-  cpu_gemm_rows(A, B, C, n, 1.0f, 0.0f, row0, row1);
+#pragma omp parallel for schedule(static)
+  for(int64_t i = row0; i < row1; ++i) {
+    for(int64_t j0 = 0; j0 < n; j0 += LANES) {
+      const int64_t w = n - j0 < LANES ? n - j0 : LANES;
+      float acc[LANES];
+#pragma omp simd
+      for(int64_t l = 0; l < LANES; ++l) acc[l] = 0.0f;
+      for(int64_t k = 0; k < n; ++k) {
+        const float a = A[i * n + k];
+        const float* b = B + k * n + j0;
+        if(w == LANES) {
+#pragma omp simd
+          for(int64_t l = 0; l < LANES; ++l) acc[l] += a * b[l];
+        } else {
+          for(int64_t l = 0; l < w; ++l) acc[l] += a * b[l];
+        }
+      }
+      for(int64_t l = 0; l < w; ++l) C[i * n + j0 + l] = acc[l];
+    }
+  }
 }

@@ -382,6 +382,29 @@ void fill_uniform(float* p, int64_t rows, int64_t cols, int64_t ld, uint64_t see
     }
 }
 
+/* sign * exp(mu + sigma * z): z from Box-Muller on two 24-bit uniforms of
+ * the same keyed hash, u1 = (bits 40..63 + 1/2) 2^-24, u2 = (bits 16..39 +
+ * 1/2) 2^-24, each formed in float (so the + 1/2 rounds away for values of
+ * 2^23 and above, and u1 can be exactly 1), z = sqrt(-2 ln u1) cos(2 pi u2);
+ * negative when bit 0 is set.  The uniforms are bit-exact with the library;
+ * the rest is evaluated in double here and in float there, so the two agree
+ * to a few float ulps, not bit for bit.  No reference counterpart: the
+ * block-scaled FP8 workload's input generator. */
+void fill_lognormal(float* p, int64_t rows, int64_t cols, int64_t ld, uint64_t seed, uint32_t matrix, double mu,
+    double sigma) {
+  const double two_pi = 6.283185307179586476925286766559;
+  for(int64_t r = 0; r < rows; ++r)
+    for(int64_t c = 0; c < cols; ++c) {
+      const uint64_t h = keyed_hash(seed, matrix, (uint64_t)(r * cols + c));
+      const float f1 = ((float)(uint32_t)(h >> 40) + 0.5f) * 5.9604644775390625e-08f;
+      const float f2 = ((float)(uint32_t)((h >> 16) & 0xffffffu) + 0.5f) * 5.9604644775390625e-08f;
+      const double u1 = f1, u2 = f2;
+      const double z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+      const double v = exp(mu + sigma * z);
+      p[r * ld + c] = (float)((h & 1u) ? -v : v);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* 6. Number formats                                                         */
 /* ------------------------------------------------------------------------ */

@@ -64,6 +64,8 @@ _sig("gemm_f64", None, ctypes.c_int, ctypes.c_int, _I64, _I64, _I64, ctypes.c_do
 _sig("keyed_hash", ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64)
 _sig("splitmix64_mix", ctypes.c_uint64, ctypes.c_uint64)
 _sig("fill_uniform", None, _F32P, _I64, _I64, _I64, ctypes.c_uint64, ctypes.c_uint32)
+_sig("fill_lognormal", None, _F32P, _I64, _I64, _I64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_double,
+     ctypes.c_double)
 _sig("bf16_rne", ctypes.c_uint16, ctypes.c_float)
 _sig("split_bf16", None, _F32P, _U16P, _U16P, _I64)
 _sig("e4m3_rne_sat", ctypes.c_uint8, ctypes.c_float)
@@ -256,6 +258,12 @@ def fill_uniform(rows, cols, seed, matrix):
     return out.reshape(rows, cols)
 
 
+def fill_lognormal(rows, cols, seed, matrix, mu=0.0, sigma=2.0):
+    out = _z(rows * cols)
+    _lib.fill_lognormal(out, rows, cols, cols, seed, matrix, mu, sigma)
+    return out.reshape(rows, cols)
+
+
 # ---- number formats -----------------------------------------------------------
 def split_bf16(x):
     x = np.ascontiguousarray(x, np.float32).ravel()
@@ -313,4 +321,9 @@ def cpu_baseline():
         _base.cpu_syrk_rows.restype = None
         _base.cpu_syr2k_rows.argtypes = [_F32P, _F32P, _F32P, _I64, ctypes.c_float, ctypes.c_float, _I64, _I64]
         _base.cpu_syr2k_rows.restype = None
+        _base.cpu_matmul_rows.argtypes = [_F32P, _F32P, _F32P, _I64, _I64, _I64]
+        _base.cpu_matmul_rows.restype = None
+        _base.cpu_gemm_mnk_rows.argtypes = [_F32P, _F32P, _F32P, _I64, _I64, _I64, ctypes.c_float, ctypes.c_float,
+                                            _I64, _I64]
+        _base.cpu_gemm_mnk_rows.restype = None
     return _base

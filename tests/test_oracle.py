@@ -278,6 +278,87 @@ def test_cpu_baseline_computes_reference_gemm(oracle):
     assert oracle.percent_diff_count(want2, got2) == 0
 
 
+def test_cpu_baseline_syr2k_rows(oracle):
+    """cpu_syr2k_rows follows polybench/syr2k.cpp:78-87: the reference's own
+    inputs pass its verify rule against the restated serial loop, and seeded
+    non-symmetric inputs with alpha != 1, beta != 1 match the FP64 form
+    alpha (A B^T + B A^T) + beta C on a partial row range (a dropped term,
+    a transposed operand or a missing alpha / beta fails)."""
+    base = oracle.cpu_baseline()
+    n = 80
+    A, B, C, want = oracle.syr2k_reference(n)
+    got = C.copy()
+    base.cpu_syr2k_rows(A.ravel(), B.ravel(), got.ravel(), n, 1.0, 1.0, 0, n)
+    assert oracle.percent_diff_count(want, got) == 0
+    n = 96
+    A, B, C = (oracle.fill_uniform(n, n, 5, i) for i in range(3))
+    got = C.copy()
+    r0, r1 = 17, 70
+    base.cpu_syr2k_rows(A.ravel(), B.ravel(), got.ravel(), n, 1.5, 1.2, r0, r1)
+    a, b, c = A.astype(np.float64), B.astype(np.float64), C.astype(np.float64)
+    ref = 1.5 * (a @ b.T + b @ a.T) + 1.2 * c
+    assert oracle.rel_frobenius(got[r0:r1], ref[r0:r1]) < 1e-6
+    np.testing.assert_array_equal(got[:r0], C[:r0])  # rows outside the range untouched
+    np.testing.assert_array_equal(got[r1:], C[r1:])
+
+
+def test_cpu_baseline_matmul_rows(oracle):
+    """cpu_matmul_rows is runtime/matmulchain.cpp:20-28's c = a b (with a
+    float accumulator): exact on the reference's identity chain, and equal
+    to the FP64 product of seeded inputs on a partial row range."""
+    base = oracle.cpu_baseline()
+    n = 64
+    eye = oracle.matmulchain_identity(n)
+    got = np.full((n, n), 7.0, np.float32)
+    base.cpu_matmul_rows(eye.ravel(), eye.ravel(), got.ravel(), n, 0, n)
+    np.testing.assert_array_equal(got, oracle.matmulchain_multiply_int(eye, eye))
+    n = 96
+    A, B = oracle.fill_uniform(n, n, 9, 0), oracle.fill_uniform(n, n, 9, 1)
+    got = np.full((n, n), np.nan, np.float32)
+    base.cpu_matmul_rows(A.ravel(), B.ravel(), got.ravel(), n, 8, 40)
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    assert oracle.rel_frobenius(got[8:40], ref[8:40]) < 1e-6
+    assert np.isnan(got[:8]).all() and np.isnan(got[40:]).all()
+
+
+@pytest.mark.parametrize("m,n,k,beta", [(70, 45, 33, 1.2), (17, 130, 100, 0.0), (64, 16, 1, 2.0)])
+def test_cpu_baseline_gemm_mnk_rows(oracle, m, n, k, beta):
+    """cpu_gemm_mnk_rows: polybench/gemm.cpp:82-91's body on an m x n x k
+    problem; beta == 0 writes C without reading it (a NaN-filled C stays
+    out of the result)."""
+    base = oracle.cpu_baseline()
+    A, B = oracle.fill_uniform(m, k, 4, 0), oracle.fill_uniform(k, n, 4, 1)
+    C = oracle.fill_uniform(m, n, 4, 2) if beta else np.full((m, n), np.nan, np.float32)
+    got = C.copy()
+    r0, r1 = 3, m - 2
+    base.cpu_gemm_mnk_rows(A.ravel(), B.ravel(), got.ravel(), m, n, k, 1.5, beta, r0, r1)
+    ref = 1.5 * (A.astype(np.float64) @ B.astype(np.float64)) + (beta * C.astype(np.float64) if beta else 0.0)
+    assert oracle.rel_frobenius(got[r0:r1], ref[r0:r1]) < 1e-6
+    np.testing.assert_array_equal(got[:r0], C[:r0])
+    np.testing.assert_array_equal(got[r1:], C[r1:])
+
+
+def test_lognormal_fill_distribution(oracle):
+    """The FP8 workload's sign * lognormal(mu, sigma) inputs: deterministic
+    per (seed, matrix), distinct across them, signs balanced, and ln|x|
+    normal with the requested mean and deviation (bounds of about five
+    standard errors for 2^18 samples; a wrong sigma, mu or sign rule fails)."""
+    rows, cols = 512, 512
+    for mu, sigma in ((0.0, 2.0), (1.5, 0.5)):
+        x = oracle.fill_lognormal(rows, cols, 7, 0, mu, sigma).astype(np.float64)
+        np.testing.assert_array_equal(x, oracle.fill_lognormal(rows, cols, 7, 0, mu, sigma))
+        assert not np.array_equal(x, oracle.fill_lognormal(rows, cols, 7, 1, mu, sigma))
+        assert not np.array_equal(x, oracle.fill_lognormal(rows, cols, 8, 0, mu, sigma))
+        n = x.size
+        assert abs((x < 0).mean() - 0.5) < 5 * 0.5 / np.sqrt(n)
+        z = np.log(np.abs(x))
+        assert abs(z.mean() - mu) < 5 * sigma / np.sqrt(n)
+        assert abs(z.std() - sigma) < 5 * sigma / np.sqrt(2 * n)
+        # Box-Muller normals: about 68.3 % within one deviation, 95.4 % within two
+        t = np.abs(z - mu) / sigma
+        assert abs((t < 1).mean() - 0.6827) < 0.005 and abs((t < 2).mean() - 0.9545) < 0.003
+
+
 # ---- covariance / correlation (polybench/covariance.cpp, polybench/correlation.cpp) ----
 
 def _stats_f64(x, n_const, correlation, eps=0.005):
