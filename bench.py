@@ -190,7 +190,7 @@ def grouped_workload(torch, mfx, h, steps, warmup, seed=1):
             "per_problem_launches_tflops": round(flop / (t_single / 1e3) / 1e12, 2),
             "tile_policy": {"id": policy, "large_config": big, "small_config": small, "waste_bound": bound,
                             "launch_per_level": per_level},
-            "chain_rel_err": chain_err, "sampled_rel_err_max": worst}
+            "chain_rel_err": chain_err, "sampled_rel_err_max": worst}, dims
 
 
 def _time_ms(torch, fn, reps):
@@ -335,36 +335,171 @@ def measure_configs(torch, mfx, h, reps=3):
     return out
 
 
-def cpu_baseline(n, target_s):
-    """OpenMP restatement of polybench/gemm.cpp:82-91 on the host cores, timed
-    on rows [0, R) of the same N x N problem (R sized to ~target_s)."""
-    import numpy as np
+def _cpu_lib():
     sys.path.insert(0, ROOT)
     from oracle import oracle
-    base = oracle.cpu_baseline()
-    threads = base.cpu_threads()
-    A = oracle.fill_uniform(n, n, 1, 0)
-    B = oracle.fill_uniform(n, n, 1, 1)
-    C = oracle.fill_uniform(n, n, 1, 2)
-    a, b, c = A.ravel(), B.ravel(), C.ravel()
-    rows = threads
+    return oracle, oracle.cpu_baseline()
+
+
+def _timed_rows(run_rows, total_rows, target_s, threads):
+    """Runs run_rows(r0, r1) on rows [0, R) of a problem, R grown from one row
+    per thread until about target_s seconds of work (at most total_rows).
+    Returns (R, seconds)."""
+    rows = min(threads, total_rows)
     t0 = time.perf_counter()
-    base.cpu_gemm_rows(a, b, c, n, 1.5, 1.2, 0, rows)
+    run_rows(0, rows)
     dt = time.perf_counter() - t0
-    per_row = dt / rows
-    more = int(max(0, target_s - dt) / per_row) // threads * threads
-    more = min(more, n - rows)
+    more = int(max(0.0, target_s - dt) / (dt / rows)) // threads * threads
+    more = min(more, total_rows - rows)
     if more > 0:
         t1 = time.perf_counter()
-        base.cpu_gemm_rows(a, b, c, n, 1.5, 1.2, rows, rows + more)
+        run_rows(rows, rows + more)
         dt += time.perf_counter() - t1
         rows += more
-    flops = 2.0 * rows * n * n
-    del A, B, C
+    return rows, dt
+
+
+_CPU_KIND = ("OpenMP restatement of the reference SYCL kernel body, one work-item per (i, j) with a serial k "
+             "loop, -O3 -ffast-math")
+
+
+def _cpu_result(flops, dt, threads, sample):
     return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"rows 0..{rows - 1} of the N={n} FP32 gemm (C=1.5*A*B+1.2*C, same seeded inputs), "
-                      f"{flops:.3e} flop in {dt:.1f} s; OpenMP restatement of the reference SYCL kernel body "
-                      f"(polybench/gemm.cpp:82-91), -O3 -ffast-math"}
+            "sample": sample + f"; {flops:.3e} flop in {dt:.1f} s; {_CPU_KIND}"}
+
+
+def cpu_baseline(n, target_s):
+    """polybench/gemm.cpp:82-91 on the host cores, timed on rows [0, R) of
+    the same N x N problem (R sized to ~target_s)."""
+    oracle, base = _cpu_lib()
+    threads = base.cpu_threads()
+    A, B, C = (oracle.fill_uniform(n, n, 1, i) for i in range(3))
+    a, b, c = A.ravel(), B.ravel(), C.ravel()
+    rows, dt = _timed_rows(lambda r0, r1: base.cpu_gemm_rows(a, b, c, n, 1.5, 1.2, r0, r1), n, target_s, threads)
+    return _cpu_result(2.0 * rows * n * n, dt, threads,
+                       f"rows 0..{rows - 1} of the N={n} FP32 gemm (C=1.5*A*B+1.2*C, same seeded inputs, "
+                       f"polybench/gemm.cpp:82-91)")
+
+
+def cpu_baselines_all(target_s):
+    """The reference kernel bodies of every other workload on the host cores,
+    each on a bounded row sample of the same-size problem with the same
+    seeded inputs (the reference's --device=cpu path, restated)."""
+    import numpy as np
+    oracle, base = _cpu_lib()
+    t = base.cpu_threads()
+    out = {}
+
+    def seeded(rows, cols, mid, seed=1):
+        return oracle.fill_uniform(rows, cols, seed, mid).ravel()
+
+    # gemm N=1024 (polybench/gemm.cpp:82-91): the whole problem, repeated to ~target_s
+    n = 1024
+    a, b, c0 = seeded(n, n, 0), seeded(n, n, 1), seeded(n, n, 2)
+    c = c0.copy()
+    reps, t0 = 0, time.perf_counter()
+    while reps == 0 or time.perf_counter() - t0 < min(target_s, 3.0):
+        base.cpu_gemm_rows(a, b, c, n, 1.5, 1.2, 0, n)
+        reps += 1
+    dt = time.perf_counter() - t0
+    out["gemm_1024_fp32"] = _cpu_result(reps * 2.0 * n ** 3, dt, t,
+                                        f"the whole N=1024 gemm, {reps} calls (polybench/gemm.cpp:82-91)")
+    log(f"cpu: gemm_1024 {out['gemm_1024_fp32']['value']:.4f} TFLOP/s")
+
+    # 2mm N=4096, D = 1.5 (A B) C + 1.2 D (polybench/2mm.cpp:99-122, two kernels): rows [0, R) of
+    # T = 1.5 A B, then of D = T C + 1.2 D (row i of D needs only row i of T)
+    n = 4096
+    a, b, cm, d = (seeded(n, n, i) for i in range(4))
+    tm = np.zeros(n * n, np.float32)
+
+    def mm2(r0, r1):
+        base.cpu_gemm_rows(a, b, tm, n, 1.5, 0.0, r0, r1)
+        base.cpu_gemm_rows(tm, cm, d, n, 1.0, 1.2, r0, r1)
+    rows, dt = _timed_rows(mm2, n, target_s, t)
+    out["2mm_4096"] = _cpu_result(4.0 * rows * n * n, dt, t, f"rows 0..{rows - 1} of both products of 2mm N={n}")
+    log(f"cpu: 2mm_4096 {out['2mm_4096']['value']:.4f} TFLOP/s")
+
+    # 3mm N=4096, G = (A B)(C D) (polybench/3mm.cpp:120-157, three kernels): rows [0, R) of E = A B,
+    # F = C D and G = E F; G's right operand holds only the sampled rows of F (the loop's cost does
+    # not depend on the values)
+    e, f, g = (np.zeros(n * n, np.float32) for _ in range(3))
+
+    def mm3(r0, r1):
+        base.cpu_matmul_rows(a, b, e, n, r0, r1)
+        base.cpu_matmul_rows(cm, d, f, n, r0, r1)
+        base.cpu_matmul_rows(e, f, g, n, r0, r1)
+    rows, dt = _timed_rows(mm3, n, target_s, t)
+    out["3mm_4096"] = _cpu_result(6.0 * rows * n * n, dt, t,
+                                  f"rows 0..{rows - 1} of the three products of 3mm N={n} (G's right operand "
+                                  f"holds the sampled rows of F only)")
+    log(f"cpu: 3mm_4096 {out['3mm_4096']['value']:.4f} TFLOP/s")
+    del a, b, cm, d, tm, e, f, g
+
+    # syrk / syr2k N=M=8192 (polybench/syrk.cpp:74-83, syr2k.cpp:78-87): the reference computes the
+    # full square, so its own work is 2 N M (syr2k 4 N M) per row
+    n = 8192
+    a, b, c = seeded(n, n, 0), seeded(n, n, 1), seeded(n, n, 2)
+    rows, dt = _timed_rows(lambda r0, r1: base.cpu_syrk_rows(a, c, n, 1.5, 1.2, r0, r1), n, target_s, t)
+    out["syrk_8192"] = _cpu_result(2.0 * rows * n * n, dt, t,
+                                   f"rows 0..{rows - 1} of the full-square syrk N=M={n} (the reference's own work)")
+    log(f"cpu: syrk_8192 {out['syrk_8192']['value']:.4f} TFLOP/s")
+    rows, dt = _timed_rows(lambda r0, r1: base.cpu_syr2k_rows(a, b, c, n, 1.5, 1.2, r0, r1), n, target_s, t)
+    out["syr2k_8192"] = _cpu_result(4.0 * rows * n * n, dt, t,
+                                    f"rows 0..{rows - 1} of the full-square syr2k N=M={n} (the reference's own work)")
+    log(f"cpu: syr2k_8192 {out['syr2k_8192']['value']:.4f} TFLOP/s")
+    del a, b, c
+
+    # matmulchain 8 x 2048^2 (runtime/matmulchain.cpp:20-28, FP32 accumulator): rows [0, R) of each
+    # of the 7 tree products; later levels' right operands hold the sampled rows only
+    n = 2048
+    mats = [seeded(n, n, 10000 + j) for j in range(8)]
+    inter = [np.zeros(n * n, np.float32) for _ in range(7)]
+    pairs = [(mats[0], mats[1]), (mats[2], mats[3]), (mats[4], mats[5]), (mats[6], mats[7]),
+             (inter[0], inter[1]), (inter[2], inter[3]), (inter[4], inter[5])]
+
+    def chain(r0, r1):
+        for (x, y), z in zip(pairs, inter):
+            base.cpu_matmul_rows(x, y, z, n, r0, r1)
+    rows, dt = _timed_rows(chain, n, target_s, t)
+    out["matmulchain_8x2048"] = _cpu_result(7 * 2.0 * rows * n * n, dt, t,
+                                            f"rows 0..{rows - 1} of each of the 7 products of the 8-long chain, "
+                                            f"N={n}")
+    log(f"cpu: matmulchain {out['matmulchain_8x2048']['value']:.4f} TFLOP/s")
+    del mats, inter, pairs
+
+    # the N=16384 contraction of the FP8 workload, in FP32 (the reference has no FP8 type): rows [0, R)
+    n = 16384
+    a = oracle.fill_lognormal(n, n, 7, 0, 0.0, 2.0).ravel()
+    b = oracle.fill_lognormal(n, n, 7, 1, 0.0, 2.0).ravel()
+    c = np.zeros(n * n, np.float32)
+    rows, dt = _timed_rows(lambda r0, r1: base.cpu_gemm_rows(a, b, c, n, 1.0, 0.0, r0, r1), n, target_s, t)
+    out["gemm_16384"] = _cpu_result(2.0 * rows * n * n, dt, t,
+                                    f"rows 0..{rows - 1} of the N={n} FP32 gemm on the same sign x lognormal(0, 2) "
+                                    f"inputs")
+    log(f"cpu: gemm_16384 {out['gemm_16384']['value']:.4f} TFLOP/s")
+    return out
+
+
+def cpu_baseline_grouped(dims, target_s):
+    """The batched workload as the reference would run it, one gemm kernel per
+    problem (polybench/gemm.cpp:82-91's body on m x n x k): every 8th of the
+    512 problems in full until ~target_s, same seeded inputs."""
+    oracle, base = _cpu_lib()
+    t = base.cpu_threads()
+    flops, done, dt = 0.0, 0, 0.0
+    for i in range(0, len(dims), 8):
+        m, n, k = (int(v) for v in dims[i])
+        a = oracle.fill_uniform(m, k, 1, 3 * i).ravel()
+        b = oracle.fill_uniform(k, n, 1, 3 * i + 1).ravel()
+        c = oracle.fill_uniform(m, n, 1, 3 * i + 2).ravel()
+        t0 = time.perf_counter()
+        base.cpu_gemm_mnk_rows(a, b, c, m, n, k, 1.0, 0.0, 0, m)
+        dt += time.perf_counter() - t0
+        flops += 2.0 * m * n * k
+        done += 1
+        if dt > target_s:
+            break
+    return _cpu_result(flops, dt, t, f"{done} of the 512 uneven problems (every 8th, whole problems)")
 
 
 def main():
@@ -471,7 +606,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16",
+        "dtype": "split-bf16",
         "precision": "split-bf16: hi/lo RNE BF16, 3 MFMA products, FP32 accumulate",
         "data": "synthetic",
         "config": {"workload": f"gemm N={n} split-BF16, C = 1.5*A*B + 1.2*C, A/B/C seeded U[-1,1) (seed 1), "
@@ -487,6 +622,8 @@ def main():
         "other_precisions": others,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_BF16, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_BF16, 4), "traffic": measured_traffic(n, "gemm_split_bf16_mfma"),
+                     "traffic_source": f"profiles/gemm_n{n}_traffic.json: rocprofv3 FETCH_SIZE / WRITE_SIZE of this "
+                                       f"kernel at this size, counter-calibrated; committed, not measured in this run",
                      "kernel": "gemm_split_bf16_mfma", "kernel_ms": round(k_avg, 4),
                      "achieved_definition": "3 BF16 MFMA products x 2N^3 flop / average kernel duration"},
     }
@@ -500,16 +637,34 @@ def main():
     log(f"measured ceilings: {cal}")
     if args.all_configs:
         out["configs"] = measure_configs(torch, mfx, h)
+    dims = None
     if not args.no_grouped:
-        g = grouped_workload(torch, mfx, h, max(3, args.steps), max(1, args.warmup))
+        g, dims = grouped_workload(torch, mfx, h, max(3, args.steps), max(1, args.warmup))
         g["frac_of_large_tile"] = round(g["tflops"] / value, 3)
         out["grouped"] = g
         log(f"grouped: {g}")
     if not args.no_cpu:
         log("CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
+        if dims is not None:
+            out["grouped"]["cpu_baseline"] = cpu_baseline_grouped(dims, args.cpu_seconds / 2)
+        if args.all_configs:
+            cpu = cpu_baselines_all(args.cpu_seconds / 2)
+            # each GPU record beside the CPU rate of the same workload
+            for name, rec in out["configs"].items():
+                key = next((k for k in cpu if name.startswith(k)), None)
+                if key is not None:
+                    rec["cpu_baseline_tflops"] = round(cpu[key]["value"], 5)
+                    rec["vs_cpu"] = round(rec["tflops"] / cpu[key]["value"], 1)
+            out["configs_cpu_baseline"] = cpu
+    # the headline is quoted at <= 1e-5 relative Frobenius error for both methods
+    accuracy_ok = errs["split-bf16"] <= 1e-5 and errs["fp32"] <= 1e-5
+    out["accuracy_ok"] = accuracy_ok
     print(json.dumps(out), flush=True)
     h.close()
+    if not accuracy_ok:
+        log(f"accuracy bar missed: split-bf16 {errs['split-bf16']:.3e}, fp32 {errs['fp32']:.3e} (bar 1e-5)")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

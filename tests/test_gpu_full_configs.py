@@ -137,7 +137,7 @@ def test_batched_uneven_and_chain_in_one_graph(handle, mfx):
     sys.path.insert(0, ROOT)
     import bench
     print("batched: 512 uneven problems + 8-long chain of 2048^2, split-BF16, hipGraph", flush=True)
-    g = bench.grouped_workload(torch, mfx, handle, 3, 1)
+    g, _ = bench.grouped_workload(torch, mfx, handle, 3, 1)
     print(f"  {g['ms']} ms, {g['tflops']} TFLOP/s, chain error {g['chain_rel_err']:.3e}, "
           f"sampled error max {g['sampled_rel_err_max']:.3e}", flush=True)
     # three chained levels
