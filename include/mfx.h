@@ -80,6 +80,9 @@ mfx_status mfx_device_name(mfx_handle handle, char* buf, size_t len);
 
 /* ---- dense contractions -------------------------------------------------- */
 
+/* Size of the fixed device array behind mfx_gemm_noalloc (one per device). */
+#define MFX_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
 /*
  * C = alpha * op(A) * op(B) + beta * C, op(A) is m x k, op(B) is k x n.
  * beta == 0 means C is not read (BLAS convention).
@@ -90,6 +93,23 @@ mfx_status mfx_device_name(mfx_handle handle, char* buf, size_t len);
  *           runtime/matmulchain.cpp:12-30 (multiply<T>, beta=0).
  */
 mfx_status mfx_gemm(mfx_handle handle, mfx_precision prec, mfx_op transA, mfx_op transB,
+    int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
+    const float* B, int64_t ldb, float beta, float* C, int64_t ldc, mfx_stream stream);
+
+/*
+ * mfx_gemm without any allocation: the same kernels and results as mfx_gemm
+ * (FP32, SPLIT_BF16 or BF16), with the packed operands in a fixed library
+ * array of MFX_NOALLOC_SCRATCH_BYTES per device instead of the handle's
+ * workspace.  It calls no hipMalloc / hipFree or other allocating routine,
+ * so code under graph capture, tight serving loops and harnesses that
+ * forbid allocation can call it.  A request whose packed operands exceed the
+ * array (4 (m + n) k bytes for SPLIT_BF16, half that for BF16, about) returns
+ * MFX_ERROR_NOT_SUPPORTED; FP8 precisions are not supported.  Every call on
+ * one device shares that device's array, so calls that overlap on one
+ * device race.  Its kernels are not recorded by mfx_timing_*.
+ * No reference counterpart: it is for callers that cannot allocate.
+ */
+mfx_status mfx_gemm_noalloc(mfx_handle handle, mfx_precision prec, mfx_op transA, mfx_op transB,
     int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
     const float* B, int64_t ldb, float beta, float* C, int64_t ldc, mfx_stream stream);
 

@@ -158,6 +158,16 @@ size_t dense_bf16x_workspace(const DenseOp& op, bool split, int config) {
 
 mfx_status launch_dense_bf16x(Handle* h, const DenseOp& op, bool split, int config, hipStream_t s) {
   if(config < 0 || config >= kBfConfigCount) return invalid("BF16 tile configuration out of range");
+  if(op.M == 0 || op.N == 0) return MFX_SUCCESS;
+  void* ws = nullptr;
+  MFX_CHECK(workspace(h, dense_bf16x_workspace(op, split, config), &ws));
+  return launch_dense_bf16x_ws(h, op, split, config, ws, s);
+}
+
+// The pack kernels write every chunk of both padded panels (zeros past the
+// matrix), so `ws` needs no clearing and may hold anything on entry.
+mfx_status launch_dense_bf16x_ws(Handle* h, const DenseOp& op, bool split, int config, void* ws, hipStream_t s) {
+  if(config < 0 || config >= kBfConfigCount) return invalid("BF16 tile configuration out of range");
   const BfConfig& c = kBfConfigs[config];
   if(op.M == 0 || op.N == 0) return MFX_SUCCESS;
   const int nhl = split ? 2 : 1;
@@ -165,8 +175,6 @@ mfx_status launch_dense_bf16x(Handle* h, const DenseOp& op, bool split, int conf
   const bool same = op.A == op.B && op.lda == op.ldb && op.a_kmajor == op.b_kmajor && op.M == op.N &&
                     pl.rows_pad_a == pl.rows_pad_b;
   if(op.nprod == 2 && pl.rows_pad_a != pl.rows_pad_b) return invalid("two-product form needs equal operand shapes");
-  void* ws = nullptr;
-  MFX_CHECK(workspace(h, dense_bf16x_workspace(op, split, config), &ws));
   u32x4* pa = static_cast<u32x4*>(ws);
   u32x4* pb = reinterpret_cast<u32x4*>(static_cast<char*>(ws) + round_up(pl.bytes_a, 256));
   // The B-role operand is indexed (n, k): B[n*ldb + k] if b_kmajor else B[k*ldb + n].

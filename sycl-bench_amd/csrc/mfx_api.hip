@@ -353,6 +353,7 @@ mfx_status mfx_create(mfx_handle* handle, int device) {
     set_last_error(std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950 only");
     return MFX_ERROR_NO_DEVICE;
   }
+  MFX_CHECK(noalloc_init(device));
   auto* hs = new(std::nothrow) mfx_handle_s;
   if(!hs) return MFX_ERROR_ALLOC;
   hs->h.device = device;

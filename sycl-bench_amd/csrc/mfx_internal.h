@@ -69,6 +69,10 @@ struct Handle {
   int* error_flag = nullptr;
 };
 
+// Looks up the device's fixed no-allocation scratch array (mfx_create calls
+// it with the device current).
+mfx_status noalloc_init(int device);
+
 // Returns a workspace pointer of at least `bytes` (grows the arena; growing
 // synchronises the device, so it must not happen under graph capture).
 mfx_status workspace(Handle* h, size_t bytes, void** out);
@@ -328,6 +332,8 @@ struct DenseOp {
 
 mfx_status launch_dense_f32(Handle* h, const DenseOp& op, int config, hipStream_t s);
 mfx_status launch_dense_bf16x(Handle* h, const DenseOp& op, bool split, int config, hipStream_t s);
+// The same on a caller-provided workspace of dense_bf16x_workspace() bytes.
+mfx_status launch_dense_bf16x_ws(Handle* h, const DenseOp& op, bool split, int config, void* ws, hipStream_t s);
 mfx_status launch_dense_fp8(Handle* h, const DenseOp& op, bool two_term, int config, hipStream_t s);
 size_t dense_bf16x_workspace(const DenseOp& op, bool split, int config);
 size_t dense_fp8_workspace(const DenseOp& op, bool two_term, int config);

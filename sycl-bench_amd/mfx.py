@@ -51,6 +51,7 @@ SIGNATURES = {
     "mfx_last_error": (ctypes.c_char_p, []),
     "mfx_device_name": (_ST, [_V, ctypes.c_char_p, ctypes.c_size_t]),
     "mfx_gemm": (_ST, [_V, _ST, _ST, _ST, _I64, _I64, _I64, _F, _V, _I64, _V, _I64, _F, _V, _I64, _V]),
+    "mfx_gemm_noalloc": (_ST, [_V, _ST, _ST, _ST, _I64, _I64, _I64, _F, _V, _I64, _V, _I64, _F, _V, _I64, _V]),
     "mfx_syrk": (_ST, [_V, _ST, _ST, ctypes.c_int, _I64, _I64, _F, _V, _I64, _F, _V, _I64, _V]),
     "mfx_syr2k": (_ST, [_V, _ST, _ST, ctypes.c_int, _I64, _I64, _F, _V, _I64, _V, _I64, _F, _V, _I64, _V]),
     "mfx_gemm_grouped": (_ST, [_V, _ST, ctypes.POINTER(Problem), _I32, ctypes.POINTER(Dep), _I32, _V]),
@@ -169,6 +170,15 @@ class Handle:
         k = A.shape[0] if transA else A.shape[1]
         _check(self.lib.mfx_gemm(self.h, prec_id(prec), int(transA), int(transB), m, n, k, alpha, _ptr(A), _ld(A),
                                  _ptr(B), _ld(B), beta, _ptr(C), _ld(C), _stream(stream)), "mfx_gemm")
+
+    def gemm_noalloc(self, A, B, C, alpha=1.0, beta=0.0, prec="split-bf16", transA=False, transB=False,
+                     stream=None):
+        """mfx_gemm_noalloc: mfx_gemm with its packed operands in the library's fixed per-device array."""
+        m, n = C.shape
+        k = A.shape[0] if transA else A.shape[1]
+        _check(self.lib.mfx_gemm_noalloc(self.h, prec_id(prec), int(transA), int(transB), m, n, k, alpha, _ptr(A),
+                                         _ld(A), _ptr(B), _ld(B), beta, _ptr(C), _ld(C), _stream(stream)),
+               "mfx_gemm_noalloc")
 
     def syrk(self, A, C, alpha=1.0, beta=0.0, prec="fp32", uplo=LOWER, mirror=True, stream=None):
         n, k = A.shape
