@@ -167,7 +167,7 @@ mfx_status mfx_autotune_grouped(mfx_handle handle, mfx_precision prec, const mfx
     bool dup = false;
     for(int q = 0; q < p; ++q) dup = dup || pair_of(q) == pair_of(p);
     // without dependencies the per-level form is the same single launch
-    if(dup || (ndeps == 0 && std::get<3>(pair_of(p)))) continue;
+    if(dup || (ndeps == 0 && std::get<3>(pair_of(p)) == 1)) continue;
     h->grouped_policy = p;
     double ms = 0.0;
     st = time_ms(s, 3, [&] { return run_grouped(h, prec, problems, count, deps, ndeps, s); }, &ms);

@@ -42,7 +42,11 @@ constexpr int GROUPED_MAX_DEPS = 4;
 
 constexpr int PACK_KBLOCKS = 32;  // 16-wide k blocks per pack item (512 k)
 
-enum GroupedItemType : int { ITEM_PACK_A = 0, ITEM_PACK_B = 1, ITEM_TILE = 2 };
+// ITEM_PREPACK (split / single BF16, one launch per segment): pack blocks
+// [x, x + y) of the segment's pre-pack job list -- operands of problems that
+// the NEXT segment's launch contracts, so no item of this launch waits for it
+// and the kernel boundary orders it before its readers.
+enum GroupedItemType : int { ITEM_PACK_A = 0, ITEM_PACK_B = 1, ITEM_TILE = 2, ITEM_PREPACK = 3 };
 
 struct GProblem {
   int M, N, K;
@@ -83,15 +87,26 @@ struct GItem {
   int type, problem, x, y;  // TILE: (x, y) = (tm, tn); PACK: x = 32-row block, y = k slice
 };
 
+// Work queues per launch: one per XCD.  Workgroups are dispatched
+// round-robin over the 8 XCDs, so workgroup b drains queue b % 8 first (the
+// planner puts all tiles of a problem in one queue, so the tiles that share
+// an operand panel run on one XCD and meet that panel in its L2), then the
+// other queues in turn.  A launch whose items wait on each other in-kernel
+// puts every item in one queue, which keeps the queue order.
+constexpr int GROUPED_QUEUES = 8;
+
 struct GroupedArgs {
   const GProblem* problems;
   const GItem* items;
   int nitems;
-  int* head;        // next item to take
+  int* head;        // GROUPED_QUEUES counters: items taken from each queue
+  const int* qbeg;  // queue q holds items [qbeg[q], qbeg[q + 1])
   int* error;       // set to 1 when a wait times out
   int* done;        // per problem: finished tiles
   int* pack_ctr;    // per packed panel: finished pack items
   int spin_limit;
+  const PackJob* prejobs;  // ITEM_PREPACK: the segment's pre-pack jobs (blocks numbered from 0)
+  int nprejobs;
 };
 
 // Tile geometry of configuration C (the same ids as the single-problem
@@ -181,15 +196,67 @@ __device__ __forceinline__ void signal(int* ctr) {
   }
 }
 
-// Next item index for the whole workgroup.  The value is the same in every
-// lane; reading it through readfirstlane lets the compiler keep the item and
-// its problem in scalar registers, leaving the vector registers to the tile.
-__device__ __forceinline__ int take_item(int* head, int* slot) {
+// Next item index for the whole workgroup, -1 when every queue is empty.
+// `tried` (LDS, zero at kernel start) counts the queues this workgroup has
+// found empty: it drains its own XCD's queue, then the next, and so on.
+// The value is the same in every lane; reading it through readfirstlane lets
+// the compiler keep the item and its problem in scalar registers, leaving
+// the vector registers to the tile.
+__device__ __forceinline__ int take_item(const GroupedArgs& g, int* slot, int* tried) {
   __syncthreads();  // everyone is done with the previous item (and its LDS)
-  if(threadIdx.x == 0) *slot = atomicAdd(head, 1);
+  if(threadIdx.x == 0) {
+    int it = -1;
+    while(*tried < GROUPED_QUEUES) {
+      const int q = (int)((blockIdx.x + *tried) % GROUPED_QUEUES);
+      const int len = g.qbeg[q + 1] - g.qbeg[q];
+      if(len > 0) {
+        const int k = atomicAdd(&g.head[q], 1);
+        if(k < len) {
+          it = g.qbeg[q] + k;
+          break;
+        }
+      }
+      ++*tried;
+    }
+    *slot = it;
+  }
   __syncthreads();
   return __builtin_amdgcn_readfirstlane(*slot);
 }
+
+// A grouped launch prepared on the host: tables, workspace layout and the
+// arguments of the persistent kernel.
+struct GroupedPlan {
+  mfx_precision prec;
+  std::vector<GProblem> problems;
+  std::vector<GItem> items;
+  std::vector<int> qbeg;  // per segment GROUPED_QUEUES + 1 queue bounds (relative to the segment's items)
+  size_t counter_ints = 0;
+  size_t off_problems = 0, off_items = 0, off_qbeg = 0, off_jobs = 0, off_counters = 0, off_packed = 0;
+  // Pack jobs: the pre-pass kernels' (jobs[0, npre)) then the pre-pack
+  // items' (jobs[npre, ...)); a job's first_block counts from the start of
+  // its segment's range.  job_op: (problem, 1 = A / 0 = B) of each job.
+  std::vector<PackJob> jobs;
+  std::vector<std::pair<int, int>> job_op;
+  int npre = 0;
+  // Launch segments: one for the whole queue, or one per dependency level
+  // (per_level), or pipelined segments.  Segment l covers items
+  // [seg_items[l], seg_items[l+1]), pre-pass jobs [seg_jobs[l],
+  // seg_jobs[l+1]) in seg_blocks[l] blocks, and its pre-pack items pack jobs
+  // [seg_prejobs[l], seg_prejobs[l+1]) (indices into jobs).
+  bool per_level = false, pipelined = false;
+  std::vector<int> seg_items, seg_jobs, seg_blocks, seg_prejobs;
+  std::vector<int> seg_big, seg_small;  // tile pair of each segment's kernel
+  const PackJob* dev_jobs = nullptr;          // the job table in device memory
+  size_t total_bytes = 0;
+  int policy = 0, big = 0, small = 0;  // tile policy and its configurations
+  bool all_vec = true;                 // every problem allows 128-bit operand loads
+  double flops = 0.0;
+};
+
+// Plans a grouped launch (host only: no device memory is touched).
+mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* problems, int count,
+    const mfx_dep* deps, int ndeps, GroupedPlan& pl);
 
 // Pre-pass: packs every job's operand (one chunk per thread).
 mfx_status launch_pack_jobs(Handle* h, int nhl, const PackJob* jobs, int njobs, int blocks, hipStream_t s);

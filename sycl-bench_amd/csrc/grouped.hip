@@ -24,7 +24,12 @@ namespace {
 struct GroupedPolicy {
   int big, small;
   double bound;
-  int per_level;  // 1: one pre-pass + persistent launch per dependency level, no in-kernel waits
+  // 0: one persistent kernel, in-kernel dependency waits; 1: one pre-pass +
+  // persistent launch per dependency level, no in-kernel waits; 2:
+  // pipelined segments -- like 1, with the independent problems spread over
+  // at least three segments and each launch packing the next segment's
+  // independent operands in pre-pack items beside its own tiles.
+  int per_level;
 };
 constexpr GroupedPolicy kPolicies[] = {
     {0, 0, 0.0},   // 0
@@ -48,6 +53,10 @@ constexpr GroupedPolicy kPolicies[] = {
     {4, 5, 1.34, 1},  // 18
     {3, 1, 1.34, 1},  // 19
     {6, 6, 0.0, 1},   // 20
+    {4, 4, 0.0, 2},   // 21: pipelined segments from here on
+    {4, 5, 1.34, 2},  // 22
+    {4, 6, 1.34, 2},  // 23
+    {3, 1, 1.34, 2},  // 24
 };
 constexpr int kPolicyCount = sizeof(kPolicies) / sizeof(kPolicies[0]);
 
@@ -167,7 +176,7 @@ int choose_grouped_policy(Handle* h, mfx_precision prec, const mfx_gemm_problem*
     // With dependency levels, one launch per level (measured faster than
     // in-kernel waits, whose agent-scope fences write back / invalidate L2);
     // without, the two modes are the same launch.
-    if(levels != (kPolicies[i].per_level != 0)) continue;
+    if(kPolicies[i].per_level == 2 || levels != (kPolicies[i].per_level != 0)) continue;
     const double e = policy_estimate(kPolicies[i], prec, problems, count, level, cus);
     if(best_est < 0 || e < best_est) {
       best_est = e;
@@ -195,29 +204,6 @@ std::pair<size_t, size_t> packed_sizes(const GProblem& P, int nhl) {
 
 }  // namespace
 
-// A grouped launch prepared on the host: tables, workspace layout and the
-// arguments of the persistent kernel.
-struct GroupedPlan {
-  mfx_precision prec;
-  std::vector<GProblem> problems;
-  std::vector<GItem> items;
-  size_t counter_ints = 0;
-  size_t off_problems = 0, off_items = 0, off_jobs = 0, off_counters = 0, off_packed = 0;
-  std::vector<std::pair<int, int>> job_list;  // (problem, 1 = A / 0 = B) packed by the pre-pass
-  std::vector<PackJob> jobs;                  // filled in bind_grouped
-  // Launch segments: one for the whole queue, or one per dependency level
-  // (per_level).  Segment l covers items [seg_items[l], seg_items[l+1]) and
-  // pre-pass jobs [seg_jobs[l], seg_jobs[l+1]) in seg_blocks[l] blocks.
-  bool per_level = false;
-  std::vector<int> seg_items, seg_jobs, seg_blocks;
-  std::vector<int> seg_big, seg_small;  // tile pair of each segment's kernel
-  const PackJob* dev_jobs = nullptr;          // the job table in device memory
-  size_t total_bytes = 0;
-  int policy = 0, big = 0, small = 0;  // tile policy and its configurations
-  bool all_vec = true;                 // every problem allows 128-bit operand loads
-  double flops = 0.0;
-};
-
 static mfx_status validate(mfx_precision prec, const mfx_gemm_problem* problems, int count, const mfx_dep* deps,
     int ndeps) {
   if(count < 0 || ndeps < 0) return invalid("negative problem or dependency count");
@@ -229,8 +215,8 @@ static mfx_status validate(mfx_precision prec, const mfx_gemm_problem* problems,
       return invalid("problem " + std::to_string(i) + ": bad dimensions");
     if((q.transA != 0 && q.transA != 1) || (q.transB != 0 && q.transB != 1))
       return invalid("problem " + std::to_string(i) + ": transA / transB must be 0 or 1");
-    const int64_t a_rows = q.transA ? q.k : q.m, a_cols = q.transA ? q.m : q.k;
-    const int64_t b_rows = q.transB ? q.n : q.k, b_cols = q.transB ? q.k : q.n;
+    const int64_t a_cols = q.transA ? q.m : q.k;
+    const int64_t b_cols = q.transB ? q.k : q.n;
     if(q.m > 0 && q.n > 0) {
       if(prec != MFX_PREC_FP32 && !bf16_k_supported(q.k, prec == MFX_PREC_SPLIT_BF16 ? 2 : 1))
         return unsupported("problem " + std::to_string(i) + ": K is beyond the packed BF16 path's limit");
@@ -239,8 +225,6 @@ static mfx_status validate(mfx_precision prec, const mfx_gemm_problem* problems,
       if(q.lda < std::max<int64_t>(1, a_cols) || q.ldb < std::max<int64_t>(1, b_cols))
         return invalid("problem " + std::to_string(i) + ": leading dimension smaller than row length");
     }
-    (void)a_rows;
-    (void)b_rows;
   }
   for(int d = 0; d < ndeps; ++d) {
     if(deps[d].consumer < 0 || deps[d].consumer >= count || deps[d].producer < 0 || deps[d].producer >= count)
@@ -250,6 +234,137 @@ static mfx_status validate(mfx_precision prec, const mfx_gemm_problem* problems,
   }
   return MFX_SUCCESS;
 }
+
+namespace {
+
+// Cost model of the pipelined planner, in milliseconds, fitted to MI355X
+// measurements of split-BF16 grouped launches: a tile of configuration c
+// takes about kTileFixedMs + kTileMsPerUnit * bm * bn * K / kTileRate[c]
+// (the 256x256 eight-wave tile at K = 2048: 0.124 ms under full load); the
+// pre-pass packs kPrepassMsPerElem per operand element over the whole chip
+// (0.53 ms for 361 M elements), and a pre-pack item costs one CU about
+// kPrepackCuMsPerElem per element (the CU reads 4 and writes 4 bytes); and
+// a launch moves at most kHbmBytesPerMs.
+constexpr double kTileFixedMs = 0.005, kTileMsPerUnit = 1.06e-9;
+constexpr double kPrepassMsPerElem = 1.47e-9, kPrepackCuMsPerElem = 1.0e-7, kHbmBytesPerMs = 5.0e9;
+constexpr int kMinPipelineSegments = 3;
+constexpr int PREPACK_BLOCKS = 32;  // 256-chunk blocks per pre-pack item (256 KiB of FP32 source)
+
+double tile_ms(int c, int64_t k) {
+  return kTileFixedMs + kTileMsPerUnit * cfg_bm(c) * cfg_bn(c) * (double)round_up(std::max<int64_t>(k, 1), 32) /
+                            kTileRate[c];
+}
+
+// Pipelined segments.  Problems with producers or consumers keep their
+// dependency level; every other problem goes, largest first, to the segment
+// where it adds least to the estimated total time.  A segment's launch takes
+// the longest of: its tile work plus the pre-pack items it runs for the next
+// segment, spread over the CUs; its longest tile; its HBM traffic (each
+// problem's packed operands read once -- its tiles share one XCD's L2 -- and
+// C written, plus 4 + 2 * planes bytes per pre-packed element).  Its
+// pre-pass comes on top: segment 0's packs all of its problems, a later
+// one only those with producers; the others were packed by the previous
+// segment's launch.
+std::vector<int> balance_segments(const std::vector<GProblem>& P, const std::vector<char>& has_prod,
+    const std::vector<char>& linked, const std::vector<int>& level, int cus, int nhl, int* nseg_out) {
+  const bool packed = nhl > 0;
+  const int count = (int)P.size();
+  int nseg = 1;
+  for(int i = 0; i < count; ++i) nseg = std::max(nseg, level[i] + 1);
+  bool any_free = false;
+  for(int i = 0; i < count; ++i) any_free = any_free || (!linked[i] && P[i].tiles_m * P[i].tiles_n > 0);
+  if(any_free) nseg = std::max(nseg, kMinPipelineSegments);
+  std::vector<double> work(nseg, 0.0), tmax(nseg, 0.0), prepass(nseg, 0.0), prepack(nseg, 0.0), bytes(nseg, 0.0);
+  std::vector<int> seg(count, 0);
+  auto elems = [&](int i) { return packed ? (double)(round_up(P[i].M, 32) + round_up(P[i].N, 32)) * P[i].kbs * 16 : 0.0; };
+  auto add = [&](int i, int s, double sign) {
+    const int tiles = P[i].tiles_m * P[i].tiles_n;
+    if(tiles == 0) return;
+    const double t = tile_ms(P[i].config, P[i].K);
+    work[s] += sign * t * tiles;
+    if(sign > 0) tmax[s] = std::max(tmax[s], t);
+    const double e = elems(i);
+    bytes[s] += sign * (e * 2 * std::max(nhl, 2) + (P[i].beta != 0.0f ? 8.0 : 4.0) * P[i].M * P[i].N);
+    if(s == 0 || has_prod[i]) {
+      prepass[s] += sign * e * kPrepassMsPerElem;
+    } else {
+      prepack[s - 1] += sign * e * kPrepackCuMsPerElem;
+      bytes[s - 1] += sign * e * (4 + 2 * nhl);
+    }
+  };
+  auto total = [&]() {
+    double t = 0.0;
+    for(int s = 0; s < nseg; ++s)
+      t += std::max(std::max((work[s] + prepack[s]) / cus, tmax[s]), bytes[s] / kHbmBytesPerMs) + prepass[s];
+    return t;
+  };
+  std::vector<int> free_list;
+  for(int i = 0; i < count; ++i) {
+    if(linked[i]) {
+      seg[i] = level[i];
+      add(i, seg[i], 1.0);
+    } else if(P[i].tiles_m * P[i].tiles_n > 0) {
+      free_list.push_back(i);
+    }
+  }
+  std::stable_sort(free_list.begin(), free_list.end(), [&](int a, int b) {
+    return tile_ms(P[a].config, P[a].K) * P[a].tiles_m * P[a].tiles_n >
+           tile_ms(P[b].config, P[b].K) * P[b].tiles_m * P[b].tiles_n;
+  });
+  for(int i : free_list) {
+    int best = 0;
+    double best_t = -1.0;
+    for(int s = 0; s < nseg; ++s) {
+      const std::vector<double> keep = tmax;
+      add(i, s, 1.0);
+      const double t = total();
+      add(i, s, -1.0);
+      tmax = keep;
+      if(best_t < 0 || t < best_t - 1e-12) {
+        best_t = t;
+        best = s;
+      }
+    }
+    seg[i] = best;
+    add(i, best, 1.0);
+  }
+  *nseg_out = nseg;
+  return seg;
+}
+
+// Splits a problem list over the GROUPED_QUEUES queues: each problem's
+// tiles go to one queue, largest total work first to the least-loaded
+// queue; a queue lists its problems in the order given.
+std::vector<std::vector<int>> queue_problems(const std::vector<int>& order, const std::vector<GProblem>& P) {
+  std::vector<int> by_work = order;
+  std::stable_sort(by_work.begin(), by_work.end(), [&](int a, int b) {
+    return tile_ms(P[a].config, P[a].K) * P[a].tiles_m * P[a].tiles_n >
+           tile_ms(P[b].config, P[b].K) * P[b].tiles_m * P[b].tiles_n;
+  });
+  std::vector<double> load(GROUPED_QUEUES, 0.0);
+  std::vector<int> q_of(P.size(), 0);
+  for(int i : by_work) {
+    const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    q_of[i] = q;
+    load[q] += tile_ms(P[i].config, P[i].K) * P[i].tiles_m * P[i].tiles_n;
+  }
+  std::vector<std::vector<int>> qs(GROUPED_QUEUES);
+  for(int i : order) qs[q_of[i]].push_back(i);
+  return qs;
+}
+
+PackJob make_job(const GProblem& P, bool is_a, int first_block) {
+  PackJob j;
+  j.src = make_pack_src(is_a ? P.A : P.B, is_a ? P.lda : P.ldb, is_a ? P.M : P.N, P.K, is_a ? P.a_kmajor : P.b_kmajor);
+  j.dst = nullptr;  // bound to the workspace by bind_grouped
+  j.kbs = P.kbs;
+  j.chunks = ceil_div(is_a ? P.M : P.N, 32) * P.kbs * 64;
+  j.blocks = (int)ceil_div(j.chunks, 256);
+  j.first_block = first_block;
+  return j;
+}
+
+}  // namespace
 
 mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* problems, int count,
     const mfx_dep* deps, int ndeps, GroupedPlan& pl) {
@@ -271,41 +386,51 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
   }
   for(int i = 0; i < count; ++i)
     for(int e = 0; e < pl.problems[i].ndeps; ++e) level[i] = std::max(level[i], level[pl.problems[i].deps[e]] + 1);
+  std::vector<char> has_prod(count), linked(count);
+  for(int i = 0; i < count; ++i) {
+    has_prod[i] = pl.problems[i].ndeps > 0;
+    linked[i] = has_prod[i] || pl.problems[i].consumed;
+  }
   pl.policy = choose_grouped_policy(h, prec, problems, count, deps, ndeps);
   const GroupedPolicy& pol = kPolicies[pl.policy];
   pl.big = grouped_config(prec, pol.big);
   pl.small = grouped_config(prec, pol.small);
   const int cus = h->cu_count > 0 ? h->cu_count : 256;
-  std::vector<int> cfg = assign_configs(pol, prec, problems, count, level, cus);
   const bool packed = prec != MFX_PREC_FP32;
   const int nhl = prec == MFX_PREC_SPLIT_BF16 ? 2 : 1;
-  size_t packed_bytes = 0;
-  int panels = 0;
   const int max_level = count ? *std::max_element(level.begin(), level.end()) : 0;
-  // One launch per level: kernel boundaries order the levels, so no problem
-  // waits or publishes in-kernel and every operand is packed by a pre-pass.
-  pl.per_level = pol.per_level && max_level > 0;
+  pl.pipelined = pol.per_level == 2;
+  // One launch per segment: kernel boundaries order the segments, so no
+  // problem waits or publishes in-kernel and every operand is packed by a
+  // pre-pass or by the previous segment's pre-pack items.
+  pl.per_level = (pol.per_level == 1 && max_level > 0) || pl.pipelined;
+  std::vector<int> cfg(count);
+  if(pl.pipelined) {
+    for(int i = 0; i < count; ++i) cfg[i] = problem_config(pol, prec, problems[i].m, problems[i].n);
+  } else {
+    cfg = assign_configs(pol, prec, problems, count, level, cus);
+  }
   // With a launch per level, a later level whose tiles leave CUs idle (the
   // last products of a chain) may take a smaller tile of the same workgroup
   // size for all its problems, in a kernel of its own tile pair, when that
   // is the least estimated time in rounds over the CUs.
   std::vector<int> level_cfg(max_level + 1, -1);
-  if(pl.per_level) {
+  if(pl.per_level && !pl.pipelined) {
     const int threads = cfg_threads(pl.big);
     for(int lv = 1; lv <= max_level; ++lv) {
-      auto level_time = [&](int c) {
+      auto level_tiles = [&](int c) {
         int64_t tiles = 0;
         for(int i = 0; i < count; ++i)
           if(level[i] == lv && problems[i].m > 0 && problems[i].n > 0)
             tiles += ceil_div(problems[i].m, cfg_bm(c >= 0 ? c : cfg[i])) * ceil_div(problems[i].n, cfg_bn(c >= 0 ? c : cfg[i]));
         return tiles;
       };
-      if(level_time(-1) >= cus) continue;
+      if(level_tiles(-1) >= cus) continue;
       double best_t = -1.0;
       int best = -1;
       for(int c = 0; c < 7; ++c) {  // tile configurations 0..6
         if(grouped_config(prec, c) != c || cfg_threads(c) != threads) continue;
-        const double t = (double)ceil_div(level_time(c), cus) * cfg_bm(c) * cfg_bn(c) / config_rate(prec, c);
+        const double t = (double)ceil_div(level_tiles(c), cus) * cfg_bm(c) * cfg_bn(c) / config_rate(prec, c);
         if(best < 0 || t < best_t) {
           best_t = t;
           best = c;
@@ -321,6 +446,8 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
     }
   }
 
+  size_t packed_bytes = 0;
+  int panels = 0;
   for(int i = 0; i < count; ++i) {
     const auto& q = problems[i];
     GProblem& P = pl.problems[i];
@@ -358,7 +485,7 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
       P.pack_slices = (int)ceil_div(P.kbs, PACK_KBLOCKS);
       packed_bytes += packed_sizes(P, nhl).first + packed_sizes(P, nhl).second;
       if(P.ndeps == 0 || pl.per_level) {
-        // packed by a pre-pass kernel before the tiles start (jobs listed below)
+        // packed by a pre-pass kernel or pre-pack items before the tiles start
         P.prepacked = 1;
       } else {
         P.pack_a_ctr = panels;
@@ -370,56 +497,142 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
     pl.flops += 2.0 * q.m * q.n * q.k;
   }
 
-  // Items level by level: the level's pack items (largest K first, so the
-  // first tiles' panels are ready first), then its tiles, largest K first, a
-  // problem's tiles row by row.  Pre-pass jobs level by level too.
+  // Segment of every problem.
+  int nseg = 1;
+  std::vector<int> seg(count, 0);
+  if(pl.pipelined) {
+    seg = balance_segments(pl.problems, has_prod, linked, level, cus, packed ? nhl : 0, &nseg);
+  } else if(pl.per_level) {
+    seg = level;
+    nseg = max_level + 1;
+  }
+
+  // Items segment by segment.  In-kernel mode (one segment): the level's
+  // pack items (largest K first, so the first tiles' panels are ready first),
+  // then its tiles, largest K first, a problem's tiles row by row, all in
+  // queue 0.  Launch-per-segment modes: each problem's tiles in one XCD
+  // queue, problems largest K first, with the segment's pre-pack items
+  // spread evenly over the queues.
+  std::vector<std::pair<int, int>> pre_ops, prepack_ops;  // (problem, is_a)
+  std::vector<int> pre_seg_end, prepack_seg_end;
   pl.seg_items.assign(1, 0);
-  pl.seg_jobs.assign(1, 0);
-  for(int lv = 0; lv <= max_level; ++lv) {
+  for(int sg = 0; sg < nseg; ++sg) {
     std::vector<int> at;
     for(int i = 0; i < count; ++i)
-      if(level[i] == lv && pl.problems[i].tiles_m * pl.problems[i].tiles_n > 0) at.push_back(i);
+      if(seg[i] == sg && pl.problems[i].tiles_m * pl.problems[i].tiles_n > 0) at.push_back(i);
     std::stable_sort(at.begin(), at.end(), [&](int a, int b) { return pl.problems[a].K > pl.problems[b].K; });
+    // operands packed by this segment's pre-pass
     for(int i : at)
-      if(pl.problems[i].prepacked && pl.problems[i].kbs > 0) {
-        pl.job_list.push_back({i, 1});
-        pl.job_list.push_back({i, 0});
+      if(pl.problems[i].prepacked && pl.problems[i].kbs > 0 && (!pl.pipelined || sg == 0 || has_prod[i])) {
+        pre_ops.push_back({i, 1});
+        pre_ops.push_back({i, 0});
       }
-    if(packed)
-      for(int i : at) {
-        const GProblem& P = pl.problems[i];
-        if(P.prepacked) continue;
-        for(int b = 0; b < (int)ceil_div(P.M, 32); ++b)
-          for(int sl = 0; sl < P.pack_slices; ++sl) pl.items.push_back({ITEM_PACK_A, i, b, sl});
-        for(int b = 0; b < (int)ceil_div(P.N, 32); ++b)
-          for(int sl = 0; sl < P.pack_slices; ++sl) pl.items.push_back({ITEM_PACK_B, i, b, sl});
+    pre_seg_end.push_back((int)pre_ops.size());
+    // operands of the next segment packed by this segment's pre-pack items
+    const int prepack_first = (int)prepack_ops.size();
+    if(pl.pipelined && packed && sg + 1 < nseg)
+      for(int i = 0; i < count; ++i)
+        if(seg[i] == sg + 1 && !has_prod[i] && pl.problems[i].kbs > 0 && pl.problems[i].tiles_m * pl.problems[i].tiles_n > 0) {
+          prepack_ops.push_back({i, 1});
+          prepack_ops.push_back({i, 0});
+        }
+    prepack_seg_end.push_back((int)prepack_ops.size());
+    const size_t first_item = pl.items.size();
+    if(!pl.per_level) {
+      // one queue, dependency level by level
+      for(int lv = 0; lv <= max_level; ++lv) {
+        if(packed)
+          for(int i : at) {
+            const GProblem& P = pl.problems[i];
+            if(level[i] != lv || P.prepacked) continue;
+            for(int b = 0; b < (int)ceil_div(P.M, 32); ++b)
+              for(int sl = 0; sl < P.pack_slices; ++sl) pl.items.push_back({ITEM_PACK_A, i, b, sl});
+            for(int b = 0; b < (int)ceil_div(P.N, 32); ++b)
+              for(int sl = 0; sl < P.pack_slices; ++sl) pl.items.push_back({ITEM_PACK_B, i, b, sl});
+          }
+        for(int i : at)
+          if(level[i] == lv)
+            for(int tm = 0; tm < pl.problems[i].tiles_m; ++tm)
+              for(int tn = 0; tn < pl.problems[i].tiles_n; ++tn) pl.items.push_back({ITEM_TILE, i, tm, tn});
       }
-    for(int i : at)
-      for(int tm = 0; tm < pl.problems[i].tiles_m; ++tm)
-        for(int tn = 0; tn < pl.problems[i].tiles_n; ++tn) pl.items.push_back({ITEM_TILE, i, tm, tn});
-    if(pl.per_level) {
-      pl.seg_items.push_back((int)pl.items.size());
-      pl.seg_jobs.push_back((int)pl.job_list.size());
-      pl.seg_big.push_back(level_cfg[lv] >= 0 ? level_cfg[lv] : pl.big);
-      pl.seg_small.push_back(level_cfg[lv] >= 0 ? level_cfg[lv] : pl.small);
+      pl.qbeg.push_back(0);
+      for(int q = 0; q < GROUPED_QUEUES; ++q) pl.qbeg.push_back((int)(pl.items.size() - first_item));
+    } else {
+      // pre-pack items: PREPACK_BLOCKS blocks of this segment's pre-pack job list each
+      int blocks = 0;
+      for(int j = prepack_first; j < (int)prepack_ops.size(); ++j) {
+        const GProblem& P = pl.problems[prepack_ops[j].first];
+        blocks += (int)ceil_div(ceil_div(prepack_ops[j].second ? P.M : P.N, 32) * P.kbs * 64, 256);
+      }
+      std::vector<GItem> pre_items;
+      for(int b = 0; b < blocks; b += PREPACK_BLOCKS)
+        pre_items.push_back({ITEM_PREPACK, 0, b, std::min(PREPACK_BLOCKS, blocks - b)});
+      const auto qs = queue_problems(at, pl.problems);
+      pl.qbeg.push_back(0);
+      size_t next_pre = 0;
+      for(int q = 0; q < GROUPED_QUEUES; ++q) {
+        std::vector<GItem> tiles;
+        for(int i : qs[q])
+          for(int tm = 0; tm < pl.problems[i].tiles_m; ++tm)
+            for(int tn = 0; tn < pl.problems[i].tiles_n; ++tn) tiles.push_back({ITEM_TILE, i, tm, tn});
+        // this queue's share of the pre-pack items, spread evenly over its tiles
+        const size_t share = (pre_items.size() * (q + 1)) / GROUPED_QUEUES - (pre_items.size() * q) / GROUPED_QUEUES;
+        const size_t nt = tiles.size();
+        size_t ti = 0;
+        for(size_t k = 0; k < share; ++k) {
+          const size_t upto = nt * (k + 1) / (share + 1);
+          while(ti < upto) pl.items.push_back(tiles[ti++]);
+          pl.items.push_back(pre_items[next_pre++]);
+        }
+        while(ti < nt) pl.items.push_back(tiles[ti++]);
+        pl.qbeg.push_back((int)(pl.items.size() - first_item));
+      }
     }
-  }
-  if(!pl.per_level) {
     pl.seg_items.push_back((int)pl.items.size());
-    pl.seg_jobs.push_back((int)pl.job_list.size());
-    pl.seg_big.push_back(pl.big);
-    pl.seg_small.push_back(pl.small);
-  } else {
-    for(auto& P : pl.problems) P.ndeps = P.consumed = 0;  // ordered by the launches
+    pl.seg_big.push_back(pl.per_level && !pl.pipelined && level_cfg[sg] >= 0 ? level_cfg[sg] : pl.big);
+    pl.seg_small.push_back(pl.per_level && !pl.pipelined && level_cfg[sg] >= 0 ? level_cfg[sg] : pl.small);
   }
-  const int segments = (int)pl.seg_items.size() - 1;
+  if(pl.per_level)
+    for(auto& P : pl.problems) P.ndeps = P.consumed = 0;  // ordered by the launches
+  const int segments = nseg;
+
+  // Job tables: pre-pass jobs (block numbers from 0 within each segment),
+  // then pre-pack jobs (likewise).
+  pl.jobs.clear();
+  pl.job_op.clear();
+  pl.seg_jobs.assign(1, 0);
+  pl.seg_blocks.assign(segments, 0);
+  for(int sg = 0, j = 0; sg < segments; ++sg) {
+    int blocks = 0;
+    for(; j < pre_seg_end[sg]; ++j) {
+      const PackJob job = make_job(pl.problems[pre_ops[j].first], pre_ops[j].second == 1, blocks);
+      blocks += job.blocks;
+      pl.jobs.push_back(job);
+      pl.job_op.push_back(pre_ops[j]);
+    }
+    pl.seg_blocks[sg] = blocks;
+    pl.seg_jobs.push_back((int)pl.jobs.size());
+  }
+  pl.npre = (int)pl.jobs.size();
+  pl.seg_prejobs.assign(1, pl.npre);
+  for(int sg = 0, j = 0; sg < segments; ++sg) {
+    int blocks = 0;
+    for(; j < prepack_seg_end[sg]; ++j) {
+      const PackJob job = make_job(pl.problems[prepack_ops[j].first], prepack_ops[j].second == 1, blocks);
+      blocks += job.blocks;
+      pl.jobs.push_back(job);
+      pl.job_op.push_back(prepack_ops[j]);
+    }
+    pl.seg_prejobs.push_back((int)pl.jobs.size());
+  }
 
   pl.off_problems = 0;
   pl.off_items = round_up(sizeof(GProblem) * std::max(1, count), 256);
-  pl.off_jobs = pl.off_items + round_up(sizeof(GItem) * std::max<size_t>(1, pl.items.size()), 256);
-  pl.off_counters = pl.off_jobs + round_up(sizeof(PackJob) * std::max<size_t>(1, pl.job_list.size()), 256);
-  // counters: error flag, one queue head per segment, per-problem done, per-panel pack counts
-  pl.counter_ints = 1 + (size_t)segments + (size_t)count + (size_t)panels;
+  pl.off_qbeg = pl.off_items + round_up(sizeof(GItem) * std::max<size_t>(1, pl.items.size()), 256);
+  pl.off_jobs = pl.off_qbeg + round_up(sizeof(int) * pl.qbeg.size(), 256);
+  pl.off_counters = pl.off_jobs + round_up(sizeof(PackJob) * std::max<size_t>(1, pl.jobs.size()), 256);
+  // counters: error flag, GROUPED_QUEUES queue heads per segment, per-problem done, per-panel pack counts
+  pl.counter_ints = 1 + (size_t)segments * GROUPED_QUEUES + (size_t)count + (size_t)panels;
   pl.off_packed = pl.off_counters + round_up(pl.counter_ints * sizeof(int), 256);
   pl.total_bytes = pl.off_packed + packed_bytes;
   return MFX_SUCCESS;
@@ -441,29 +654,10 @@ static mfx_status bind_grouped(Handle* h, GroupedPlan& pl, void* ws, GroupedArgs
       P.packed_b = reinterpret_cast<u32x4*>(base + off);
       off += sz.second;
     }
-    pl.jobs.clear();
-    pl.seg_blocks.assign(pl.seg_items.size() - 1, 0);
-    int blocks = 0, seg = 0;
-    for(size_t ji = 0; ji < pl.job_list.size(); ++ji) {
-      while((int)ji >= pl.seg_jobs[seg + 1]) {
-        pl.seg_blocks[seg++] = blocks;
-        blocks = 0;
-      }
-      const auto& jb = pl.job_list[ji];
-      const GProblem& P = pl.problems[jb.first];
-      const bool is_a = jb.second == 1;
-      PackJob j;
-      j.src = make_pack_src(is_a ? P.A : P.B, is_a ? P.lda : P.ldb, is_a ? P.M : P.N, P.K,
-          is_a ? P.a_kmajor : P.b_kmajor);
-      j.dst = is_a ? P.packed_a : P.packed_b;
-      j.kbs = P.kbs;
-      j.chunks = ceil_div(is_a ? P.M : P.N, 32) * P.kbs * 64;
-      j.blocks = (int)ceil_div(j.chunks, 256);
-      j.first_block = blocks;
-      blocks += j.blocks;
-      pl.jobs.push_back(j);
+    for(size_t j = 0; j < pl.jobs.size(); ++j) {
+      const GProblem& P = pl.problems[pl.job_op[j].first];
+      pl.jobs[j].dst = pl.job_op[j].second == 1 ? P.packed_a : P.packed_b;
     }
-    if(!pl.job_list.empty()) pl.seg_blocks[seg] = blocks;
   }
   // The previous upload from the staging buffer must have completed.
   if(h->staging_ready) MFX_HIP(hipEventSynchronize(h->staging_ready));
@@ -478,20 +672,24 @@ static mfx_status bind_grouped(Handle* h, GroupedPlan& pl, void* ws, GroupedArgs
   char* st = static_cast<char*>(h->staging);
   memcpy(st + pl.off_problems, pl.problems.data(), sizeof(GProblem) * pl.problems.size());
   memcpy(st + pl.off_items, pl.items.data(), sizeof(GItem) * pl.items.size());
+  memcpy(st + pl.off_qbeg, pl.qbeg.data(), sizeof(int) * pl.qbeg.size());
   if(!pl.jobs.empty()) memcpy(st + pl.off_jobs, pl.jobs.data(), sizeof(PackJob) * pl.jobs.size());
   MFX_HIP(hipMemcpyAsync(base, st, pl.off_counters, hipMemcpyHostToDevice, s));
   MFX_HIP(hipEventRecord(h->staging_ready, s));
   int* counters = reinterpret_cast<int*>(base + pl.off_counters);
   g.problems = reinterpret_cast<const GProblem*>(base + pl.off_problems);
   g.items = reinterpret_cast<const GItem*>(base + pl.off_items);
+  g.qbeg = reinterpret_cast<const int*>(base + pl.off_qbeg);
   pl.dev_jobs = reinterpret_cast<const PackJob*>(base + pl.off_jobs);
   g.nitems = (int)pl.items.size();
   const int segments = (int)pl.seg_items.size() - 1;
   g.error = counters;
-  g.head = counters + 1;  // segment 0; segment l uses head + l
-  g.done = counters + 1 + segments;
-  g.pack_ctr = counters + 1 + segments + pl.problems.size();
+  g.head = counters + 1;  // segment 0; segment l uses head + l * GROUPED_QUEUES
+  g.done = counters + 1 + (size_t)segments * GROUPED_QUEUES;
+  g.pack_ctr = g.done + pl.problems.size();
   g.spin_limit = h->spin_limit;
+  g.prejobs = nullptr;
+  g.nprejobs = 0;
   return MFX_SUCCESS;
 }
 
@@ -507,13 +705,16 @@ static mfx_status launch_grouped_kernels(Handle* h, const GroupedPlan& pl, const
   const int nhl = pl.prec == MFX_PREC_SPLIT_BF16 ? 2 : 1;
   const int segments = (int)pl.seg_items.size() - 1;
   for(int l = 0; l < segments; ++l) {
-    if(pl.prec != MFX_PREC_FP32 && !pl.seg_blocks.empty())
+    if(pl.prec != MFX_PREC_FP32)
       MFX_CHECK(launch_pack_jobs(h, nhl, pl.dev_jobs + pl.seg_jobs[l], pl.seg_jobs[l + 1] - pl.seg_jobs[l],
           pl.seg_blocks[l], s));
     GroupedArgs gl = g;
     gl.items = g.items + pl.seg_items[l];
     gl.nitems = pl.seg_items[l + 1] - pl.seg_items[l];
-    gl.head = g.head + l;
+    gl.head = g.head + (size_t)l * GROUPED_QUEUES;
+    gl.qbeg = g.qbeg + (size_t)l * (GROUPED_QUEUES + 1);
+    gl.prejobs = pl.dev_jobs + pl.seg_prejobs[l];
+    gl.nprejobs = pl.seg_prejobs[l + 1] - pl.seg_prejobs[l];
     if(gl.nitems == 0) continue;
     const int big = pl.seg_big[l], small = pl.seg_small[l];
     if(pl.prec == MFX_PREC_FP32) MFX_CHECK(launch_grouped_f32(h, big, small, pl.all_vec, gl, grid, s));

@@ -69,14 +69,81 @@ __global__ __launch_bounds__(256) void pack_jobs_kernel(const PackJob* jobs, int
   if(active) store_chunk<NHL>(j.dst, blk, kb, lane, j.kbs, v);
 }
 
+// Pre-pack item: chunks [b0 * 256, (b0 + nb) * 256) of the segment's pre-pack
+// job list (pack_jobs_kernel's numbering: jobs start on 256-chunk blocks, so
+// the 64 chunks of one wave and one pass slot are one (blk, kb) pair of one
+// job).  Each thread keeps U chunks' global loads in flight per pass; a
+// non-k-major source goes through the wave's own LDS tiles (U per wave, in
+// the tile's LDS, which no tile uses while the workgroup runs this item).
+template <int NHL, int NT, int U>
+__device__ __forceinline__ void prepack_item(const PackJob* jobs, int njobs, int b0, int nb, float* tiles) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  float* wtile = tiles + (tid >> 6) * U * STAGED_TILE_FLOATS;
+  const int64_t c_end = (int64_t)(b0 + nb) * 256;
+  for(int64_t base = (int64_t)b0 * 256; base < c_end; base += (int64_t)NT * U) {
+    int jix[U], blk[U], kb[U];
+    bool act[U], kmaj[U];
+    float v[U][8];
+    float4 x[U][2];
+#pragma unroll
+    for(int i = 0; i < U; ++i) {
+      const int64_t c = base + tid + i * NT;
+      // one 256-chunk block per wave and slot: the search runs on scalars
+      const int b = __builtin_amdgcn_readfirstlane((int)(c >> 8));
+      int lo = 0, hi = njobs - 1;
+      while(lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if(jobs[mid].first_block <= b) lo = mid;
+        else hi = mid - 1;
+      }
+      const PackJob& J = jobs[lo];
+      jix[i] = lo;
+      const int64_t local = c - (int64_t)J.first_block * 256;
+      act[i] = c < c_end && local < J.chunks;
+      kmaj[i] = J.src.kmajor != 0;
+      const int64_t t = local >> 6;
+      if(kmaj[i]) {
+        blk[i] = (int)(t / J.kbs);
+        kb[i] = (int)(t % J.kbs);
+        if(act[i]) load_chunk8(J.src, blk[i], kb[i], lane, v[i]);
+      } else {
+        const int nblk = (int)(J.chunks / ((int64_t)J.kbs * 64));
+        blk[i] = (int)(t % nblk);
+        kb[i] = (int)(t / nblk);
+        staged_load(J.src, blk[i], kb[i], lane, act[i], x[i]);
+      }
+    }
+#pragma unroll
+    for(int i = 0; i < U; ++i)
+      if(!kmaj[i]) staged_write(x[i], lane, wtile + i * STAGED_TILE_FLOATS);
+    __syncthreads();
+#pragma unroll
+    for(int i = 0; i < U; ++i)
+      if(!kmaj[i]) staged_read(lane, wtile + i * STAGED_TILE_FLOATS, v[i]);
+#pragma unroll
+    for(int i = 0; i < U; ++i)
+      if(act[i]) store_chunk<NHL>(jobs[jix[i]].dst, blk[i], kb[i], lane, jobs[jix[i]].kbs, v[i]);
+    __syncthreads();  // the next pass rewrites the tiles
+  }
+}
+
+// Grouped tiles mostly stream panels that no other running tile shares, so
+// they run the deeper pipeline: four buffers of half a stage, three in
+// flight, in the LDS that bf16_tile uses for two.
+constexpr int PIPE_NB = 4;
+template <int NHL>
+constexpr int pipe_kb() {
+  return kb_per_stage<NHL>() / 2;
+}
+
 template <int NHL, int C>
 __device__ __forceinline__ void run_tile(const BfTileArgs& t, int tm, int tn, u32x4* lds) {
-  bf16_tile<Geo<C>::BM, Geo<C>::BN, Geo<C>::WM, Geo<C>::WN, NHL>(t, tm, tn, lds);
+  bf16_tile_pipe<Geo<C>::BM, Geo<C>::BN, Geo<C>::WM, Geo<C>::WN, NHL, pipe_kb<NHL>(), PIPE_NB>(t, tm, tn, lds);
 }
 
 template <int NHL, int C>
 constexpr int bf16_smem_chunks() {
-  return BfCfg<Geo<C>::BM, Geo<C>::BN, Geo<C>::WM, Geo<C>::WN, NHL>::SMEM_CHUNKS;
+  return BfPipeCfg<Geo<C>::BM, Geo<C>::BN, Geo<C>::WM, Geo<C>::WN, NHL, pipe_kb<NHL>(), PIPE_NB>::SMEM_CHUNKS;
 }
 
 template <int NHL, int BIG, int SMALL>
@@ -85,11 +152,20 @@ __global__ __launch_bounds__(geo_threads<BIG>()) void grouped_bf16x_kernel(const
   static_assert(geo_threads<SMALL>() == NT, "both tiles of a policy use the same workgroup");
   constexpr int S0 = bf16_smem_chunks<NHL, BIG>(), S1 = bf16_smem_chunks<NHL, SMALL>();
   __shared__ u32x4 lds[S0 > S1 ? S0 : S1];
-  __shared__ int slot, ready;
+  __shared__ int slot, ready, tried;
+  // pre-pack chunks in flight per thread: as many per-wave LDS tiles as fit
+  constexpr int LDS_FLOATS = (S0 > S1 ? S0 : S1) * 4;
+  constexpr int U = LDS_FLOATS / (NT / 64 * STAGED_TILE_FLOATS) < 4 ? LDS_FLOATS / (NT / 64 * STAGED_TILE_FLOATS) : 4;
+  static_assert(U >= 1, "the tile's LDS holds at least one staging tile per wave");
+  if(threadIdx.x == 0) tried = 0;
   for(;;) {
-    const int it = take_item(g.head, &slot);
-    if(it >= g.nitems) break;
+    const int it = take_item(g, &slot, &tried);
+    if(it < 0) break;
     const GItem item = g.items[it];
+    if(item.type == ITEM_PREPACK) {
+      prepack_item<NHL, NT, U>(g.prejobs, g.nprejobs, item.x, item.y, reinterpret_cast<float*>(lds));
+      continue;
+    }
     const GProblem& P = g.problems[item.problem];
     const int bm = cfg_bm(P.config), bn = cfg_bn(P.config);
     if(item.type != ITEM_TILE) {

@@ -27,10 +27,11 @@ __global__ __launch_bounds__(geo_threads<BIG>()) void grouped_f32_kernel(const G
   static_assert(geo_threads<SMALL>() == geo_threads<BIG>(), "both tiles of a policy use the same workgroup");
   constexpr int S0 = f32_smem_floats<BIG>(), S1 = f32_smem_floats<SMALL>();
   __shared__ float smem[S0 > S1 ? S0 : S1];
-  __shared__ int slot, ready;
+  __shared__ int slot, ready, tried;
+  if(threadIdx.x == 0) tried = 0;
   for(;;) {
-    const int it = take_item(g.head, &slot);
-    if(it >= g.nitems) break;
+    const int it = take_item(g, &slot, &tried);
+    if(it < 0) break;
     const GItem item = g.items[it];
     const GProblem& P = g.problems[item.problem];
     if(P.ndeps > 0 && !wg_wait(&ready, [&] { return poll_producers(P, g); })) continue;

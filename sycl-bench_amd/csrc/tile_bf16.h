@@ -155,6 +155,44 @@ __device__ __forceinline__ void load_chunk8_staged(const PackSrc& s, int blk, in
   }
 }
 
+// load_chunk8_staged in two halves, so that a caller can keep the global
+// loads of several chunks in flight: staged_load issues the lane's two
+// 128-bit loads of the (blk, kb) pair's 16 x 32 source tile into registers;
+// staged_read, after the caller has written them to the wave's LDS tile with
+// staged_write and synchronised, reads the lane's 8 k values of its row.
+__device__ __forceinline__ void staged_load(const PackSrc& s, int blk, int kb, int lane, bool active, float4 (&x)[2]) {
+  const int r0 = blk * 32, k0 = kb * 16;
+#pragma unroll
+  for(int h = 0; h < 2; ++h) {
+    const int q = lane + h * 64;
+    const int gk = k0 + (q >> 3), gr = r0 + (q & 7) * 4;
+    x[h] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if(active && gk < s.K) {
+      const float* p = s.src + (int64_t)gk * s.ld + gr;
+      if(s.vec && gr + 4 <= s.rows) {
+        x[h] = *reinterpret_cast<const float4*>(p);
+      } else {
+        if(gr < s.rows) x[h].x = p[0];
+        if(gr + 1 < s.rows) x[h].y = p[1];
+        if(gr + 2 < s.rows) x[h].z = p[2];
+        if(gr + 3 < s.rows) x[h].w = p[3];
+      }
+    }
+  }
+}
+__device__ __forceinline__ void staged_write(const float4 (&x)[2], int lane, float* tile) {
+#pragma unroll
+  for(int h = 0; h < 2; ++h) {
+    const int q = lane + h * 64, k = q >> 3;
+    *reinterpret_cast<float4*>(tile + k * 32 + (k >= 8 ? 32 : 0) + (q & 7) * 4) = x[h];
+  }
+}
+__device__ __forceinline__ void staged_read(int lane, const float* tile, float (&v)[8]) {
+  const int kh = (lane >> 5) * 8, r = lane & 31;
+#pragma unroll
+  for(int e = 0; e < 8; ++e) v[e] = tile[(kh + e) * 32 + (kh ? 32 : 0) + r];
+}
+
 // Splits a loaded lane-chunk and stores it at its fragment-order position in
 // the packed operand `dst` (kbs = number of 16-wide k blocks).
 template <int NHL>
@@ -220,6 +258,40 @@ __device__ __forceinline__ void bf16_kb_mma(floatx16 (&acc)[FM][FN], const u32x4
   // Product-major order: consecutive MFMAs write different accumulators, so
   // none waits on the previous one's result.  The small cross terms go in
   // first so they are added while the accumulator is smallest.
+  if(NHL == 2) {
+#pragma unroll
+    for(int i = 0; i < FM; ++i)
+#pragma unroll
+      for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for(int i = 0; i < FM; ++i)
+#pragma unroll
+      for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+  }
+#pragma unroll
+  for(int i = 0; i < FM; ++i)
+#pragma unroll
+    for(int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+}
+
+// bf16_kb_mma for a stage of KBS k blocks (chunk (blk, kb, hl, lane) of a
+// stage at ((blk * KBS + kb) * NHL + hl) * 64 + lane).
+template <int FM, int FN, int NHL, int KBS>
+__device__ __forceinline__ void bf16_kb_mma_n(floatx16 (&acc)[FM][FN], const u32x4* sa, const u32x4* sb, int wm, int wn,
+    int lane, int kb) {
+  bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+  for(int i = 0; i < FM; ++i) {
+    const int blk = wm * FM + i;
+    ah[i] = as_bf16x8(sa[(blk * KBS + kb) * NHL * 64 + lane]);
+    if(NHL == 2) al[i] = as_bf16x8(sa[((blk * KBS + kb) * NHL + 1) * 64 + lane]);
+  }
+#pragma unroll
+  for(int j = 0; j < FN; ++j) {
+    const int blk = wn * FN + j;
+    bh[j] = as_bf16x8(sb[(blk * KBS + kb) * NHL * 64 + lane]);
+    if(NHL == 2) bl[j] = as_bf16x8(sb[((blk * KBS + kb) * NHL + 1) * 64 + lane]);
+  }
   if(NHL == 2) {
 #pragma unroll
     for(int i = 0; i < FM; ++i)
@@ -332,6 +404,106 @@ __device__ __forceinline__ void bf16_tile(const BfTileArgs& t, int tm, int tn, u
     __syncthreads();
   }
   if(steps > 0) compute((steps - 1) & 1);
+
+  static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
+    constexpr int i = decltype(ij)::value / FN, j = decltype(ij)::value % FN;
+    store_block32(t.ep, acc[i][j], tm * BM + wm * WM + i * 32, tn * BN + wn * WN + j * 32, lane);
+  });
+}
+
+// Waits until at most n of this wave's vector memory operations are
+// outstanding (s_waitcnt vmcnt(n), n <= 63; expcnt and lgkmcnt left alone).
+template <int n>
+__device__ __forceinline__ void vm_wait() {
+  static_assert(n >= 0 && n < 64, "vmcnt holds 6 bits");
+  __builtin_amdgcn_s_waitcnt((n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// bf16_tile with a deeper pipeline: NB LDS buffers of KB k blocks each, NB-1
+// stages in flight.  A tile whose panels come from HBM rather than L2 (one
+// tile per problem, as in a batch of small problems) is bound by the bytes
+// it keeps in flight: two 64 KiB stages (one in flight) stream at about
+// 3.7 TB/s over the chip, so four 32 KiB stages (three in flight) in the
+// same LDS double the bytes in flight.  The MFMA sequence -- k blocks in
+// order, products in the same order -- is that of bf16_tile, so both give
+// bit-identical results.  `lds` holds NB * (CH_A + CH_B) chunks.
+template <int BM, int BN, int WM, int WN, int NHL, int KB, int NB>
+struct BfPipeCfg {
+  static constexpr int NWM = BM / WM, NWN = BN / WN, NW = NWM * NWN, NT = NW * 64;
+  static constexpr int FM = WM / 32, FN = WN / 32;
+  static constexpr int CH_A = BM / 32 * KB * NHL * 64;  // 16-byte chunks per stage
+  static constexpr int CH_B = BN / 32 * KB * NHL * 64;
+  static constexpr int PER_A = CH_A / NT, PER_B = CH_B / NT;
+  static constexpr int SMEM_CHUNKS = NB * (CH_A + CH_B);
+  static_assert(CH_A % NT == 0 && CH_B % NT == 0, "stage must split evenly");
+  static_assert(kb_per_stage<NHL>() % KB == 0, "stages divide the packed k padding");
+  static_assert(NB >= 2 && (NB - 2) * (PER_A + PER_B) < 64, "in-flight DMA count fits vmcnt");
+};
+
+template <int BM, int BN, int WM, int WN, int NHL, int KB, int NB>
+__device__ __forceinline__ void bf16_tile_pipe(const BfTileArgs& t, int tm, int tn, u32x4* lds) {
+  using Cfg = BfPipeCfg<BM, BN, WM, WN, NHL, KB, NB>;
+  constexpr int NT = Cfg::NT, FM = Cfg::FM, FN = Cfg::FN;
+  constexpr int SA = Cfg::CH_A, SB = Cfg::CH_B;
+  constexpr int PER = Cfg::PER_A + Cfg::PER_B;  // DMA instructions per stage and wave
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm = wid / Cfg::NWN, wn = wid % Cfg::NWN;
+
+  floatx16 acc[FM][FN];
+  zero_acc(acc);
+
+  constexpr int RUN = KB * NHL * 64;
+  const int KT = t.kbs / KB;
+  const int steps = KT * t.nprod;
+  const int64_t blk_stride = (int64_t)t.kbs * NHL * 64;
+  const int wave_base = __builtin_amdgcn_readfirstlane(tid - lane);
+  int off_a[2][Cfg::PER_A], off_b[2][Cfg::PER_B];
+#pragma unroll
+  for(int p = 0; p < 2; ++p) {
+    const int last_a = (p ? t.nblk_b : t.nblk_a) - 1, last_b = (p ? t.nblk_a : t.nblk_b) - 1;
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_A; ++i) {
+      const int c = tid + i * NT;
+      const int blk = c / RUN, run = c - blk * RUN;
+      off_a[p][i] = (min(tm * (BM / 32) + blk, last_a) - tm * (BM / 32)) * (int)blk_stride + run;
+    }
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_B; ++i) {
+      const int c = tid + i * NT;
+      const int blk = c / RUN, run = c - blk * RUN;
+      off_b[p][i] = (min(tn * (BN / 32) + blk, last_b) - tn * (BN / 32)) * (int)blk_stride + run;
+    }
+  }
+  auto issue = [&](int s) __attribute__((always_inline)) {
+    const int prod = s / KT, kt = s - prod * KT;
+    const u32x4* A = (prod ? t.PB : t.PA) + ((int64_t)tm * (BM / 32) * blk_stride + (int64_t)kt * RUN);
+    const u32x4* B = (prod ? t.PA : t.PB) + ((int64_t)tn * (BN / 32) * blk_stride + (int64_t)kt * RUN);
+    u32x4* sa = lds + (s % NB) * (SA + SB);
+    u32x4* sb = sa + SA;
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_A; ++i) lds_dma16(A + (prod ? off_a[1][i] : off_a[0][i]), sa + wave_base + i * NT);
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_B; ++i) lds_dma16(B + (prod ? off_b[1][i] : off_b[0][i]), sb + wave_base + i * NT);
+  };
+
+  // Stages 0 .. NB-2 in flight before the loop; iteration s waits for
+  // stage s (its own DMA, then the barrier for everyone's), which also means
+  // every wave has finished contracting stage s-1, so stage s+NB-1 can reuse
+  // that buffer.
+#pragma unroll
+  for(int p = 0; p < NB - 1; ++p)
+    if(p < steps) issue(p);
+  for(int s = 0; s < steps; ++s) {
+    const int after = steps - 1 - s;  // stages issued after s, at most NB - 2 in flight
+    if(after >= NB - 2) vm_wait<(NB - 2) * PER>();
+    else if(NB > 3 && after == 1) vm_wait<PER>();
+    else lds_dma_wait();
+    __syncthreads();
+    if(s + NB - 1 < steps) issue(s + NB - 1);
+    const u32x4* sa = lds + (s % NB) * (SA + SB);
+#pragma unroll
+    for(int kb = 0; kb < KB; ++kb) bf16_kb_mma_n<FM, FN, NHL, KB>(acc, sa, sa + SA, wm, wn, lane, kb);
+  }
 
   static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
     constexpr int i = decltype(ij)::value / FN, j = decltype(ij)::value % FN;
