@@ -25,6 +25,7 @@ void replace_tuned(const std::map<std::string, int>& t);
 mfx_status dispatch_dense(Handle* h, mfx_precision prec, DenseOp& op, int config, hipStream_t s);
 int grouped_policy_count();
 bool grouped_policy_info(int id, int* big, int* small, double* bound, int* per_level);
+bool grouped_policy_direct(int id);
 std::string grouped_tune_key(mfx_precision p, const mfx_gemm_problem* problems, int count, const mfx_dep* deps,
     int ndeps);
 mfx_status run_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* problems, int count,
@@ -162,7 +163,8 @@ mfx_status mfx_autotune_grouped(mfx_handle handle, mfx_precision prec, const mfx
         big = big == 0 ? 3 : big;
         small = small == 0 ? 3 : small;
       }
-      return std::make_tuple(big, small, big == small ? 0.0 : bound, per_level);
+      const bool direct = prec != MFX_PREC_FP32 && grouped_policy_direct(q);
+      return std::make_tuple(big, small, big == small ? 0.0 : bound, per_level, direct);
     };
     bool dup = false;
     for(int q = 0; q < p; ++q) dup = dup || pair_of(q) == pair_of(p);

@@ -67,6 +67,7 @@ struct GProblem {
   u32x4* packed_b;
   int pack_a_ctr, pack_b_ctr; // BF16: first panel counter of A / B
   int prepacked;              // BF16: operands packed by the pre-pass kernel
+  int direct;                 // BF16: tiles read the FP32 operands directly (bf16_tile_direct), no packing
   int consumed;               // another problem waits for this one's tiles
   int ndeps;
   int deps[GROUPED_MAX_DEPS];  // producer problem indices

@@ -30,6 +30,9 @@ struct GroupedPolicy {
   // at least three segments and each launch packing the next segment's
   // independent operands in pre-pack items beside its own tiles.
   int per_level;
+  // BF16: problems whose operands allow it (K % 16 == 0, aligned rows) run
+  // bf16_tile_direct on their FP32 operands instead of being packed.
+  int direct;
 };
 constexpr GroupedPolicy kPolicies[] = {
     {0, 0, 0.0},   // 0
@@ -57,6 +60,15 @@ constexpr GroupedPolicy kPolicies[] = {
     {4, 5, 1.34, 2},  // 22
     {4, 6, 1.34, 2},  // 23
     {3, 1, 1.34, 2},  // 24
+    {4, 4, 0.0, 1, 1},   // 25: FP32 operands read directly (BF16) from here on
+    {4, 5, 1.34, 1, 1},  // 26
+    {4, 6, 1.34, 1, 1},  // 27
+    {3, 1, 1.34, 1, 1},  // 28
+    {4, 4, 0.0, 0, 1},   // 29: in-kernel waits
+    {4, 5, 1.34, 0, 1},  // 30
+    {4, 4, 0.0, 2, 1},   // 31: pipelined segments
+    {4, 6, 1.34, 2, 1},  // 32
+    {4, 5, 1.34, 2, 1},  // 33
 };
 constexpr int kPolicyCount = sizeof(kPolicies) / sizeof(kPolicies[0]);
 
@@ -135,6 +147,8 @@ double policy_estimate(const GroupedPolicy& pol, mfx_precision prec, const mfx_g
 
 int grouped_policy_count() { return kPolicyCount; }
 
+bool grouped_policy_direct(int id) { return id >= 0 && id < kPolicyCount && kPolicies[id].direct; }
+
 bool grouped_policy_info(int id, int* big, int* small, double* bound, int* per_level) {
   if(id < 0 || id >= kPolicyCount) return false;
   *big = kPolicies[id].big;
@@ -177,6 +191,9 @@ int choose_grouped_policy(Handle* h, mfx_precision prec, const mfx_gemm_problem*
     // in-kernel waits, whose agent-scope fences write back / invalidate L2);
     // without, the two modes are the same launch.
     if(kPolicies[i].per_level == 2 || levels != (kPolicies[i].per_level != 0)) continue;
+    // BF16: the direct tiles (no packing pass; measured 1.3x faster on the
+    // batched workload); FP32 has nothing to pack
+    if(prec != MFX_PREC_FP32 && !kPolicies[i].direct) continue;
     const double e = policy_estimate(kPolicies[i], prec, problems, count, level, cus);
     if(best_est < 0 || e < best_est) {
       best_est = e;
@@ -240,12 +257,13 @@ namespace {
 // Cost model of the pipelined planner, in milliseconds, fitted to MI355X
 // measurements of split-BF16 grouped launches: a tile of configuration c
 // takes about kTileFixedMs + kTileMsPerUnit * bm * bn * K / kTileRate[c]
-// (the 256x256 eight-wave tile at K = 2048: 0.124 ms under full load); the
+// (the 256x256 eight-wave tile at K = 2048: 0.2 ms under full load, with
+// its panels streamed from HBM; the 512 uneven problems: 1.0 ms); the
 // pre-pass packs kPrepassMsPerElem per operand element over the whole chip
 // (0.53 ms for 361 M elements), and a pre-pack item costs one CU about
 // kPrepackCuMsPerElem per element (the CU reads 4 and writes 4 bytes); and
 // a launch moves at most kHbmBytesPerMs.
-constexpr double kTileFixedMs = 0.005, kTileMsPerUnit = 1.06e-9;
+constexpr double kTileFixedMs = 0.005, kTileMsPerUnit = 1.67e-9;
 constexpr double kPrepassMsPerElem = 1.47e-9, kPrepackCuMsPerElem = 1.0e-7, kHbmBytesPerMs = 5.0e9;
 constexpr int kMinPipelineSegments = 3;
 constexpr int PREPACK_BLOCKS = 32;  // 256-chunk blocks per pre-pack item (256 KiB of FP32 source)
@@ -332,24 +350,44 @@ std::vector<int> balance_segments(const std::vector<GProblem>& P, const std::vec
   return seg;
 }
 
-// Splits a problem list over the GROUPED_QUEUES queues: each problem's
-// tiles go to one queue, largest total work first to the least-loaded
-// queue; a queue lists its problems in the order given.
-std::vector<std::vector<int>> queue_problems(const std::vector<int>& order, const std::vector<GProblem>& P) {
-  std::vector<int> by_work = order;
-  std::stable_sort(by_work.begin(), by_work.end(), [&](int a, int b) {
-    return tile_ms(P[a].config, P[a].K) * P[a].tiles_m * P[a].tiles_n >
-           tile_ms(P[b].config, P[b].K) * P[b].tiles_m * P[b].tiles_n;
-  });
-  std::vector<double> load(GROUPED_QUEUES, 0.0);
-  std::vector<int> q_of(P.size(), 0);
-  for(int i : by_work) {
-    const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    q_of[i] = q;
-    load[q] += tile_ms(P[i].config, P[i].K) * P[i].tiles_m * P[i].tiles_n;
+// Tiles of one problem that go to one queue: at most one round of an XCD's
+// CUs (kQueueGroupTiles), whole tile rows where possible, so the group's
+// tiles share their A panels (and B panels) in that XCD's L2.
+constexpr int kQueueGroupTiles = 32;
+struct TileGroup {
+  int problem, first, count;  // tiles [first, first + count) in row-major tile order
+};
+
+// Splits a problem list over the GROUPED_QUEUES queues: each problem's tiles
+// in groups of at most kQueueGroupTiles, groups by decreasing work to the
+// least-loaded queue; a queue lists its groups in the order of `order`
+// (largest K first), so long tiles start first everywhere.
+std::vector<std::vector<TileGroup>> queue_tiles(const std::vector<int>& order, const std::vector<GProblem>& P) {
+  std::vector<TileGroup> groups;
+  std::vector<int> rank(P.size(), 0);
+  for(size_t r = 0; r < order.size(); ++r) rank[order[r]] = (int)r;
+  for(int i : order) {
+    const int tiles = P[i].tiles_m * P[i].tiles_n;
+    const int per = std::min(tiles, kQueueGroupTiles);
+    const int ng = (tiles + per - 1) / per;
+    for(int g = 0; g < ng; ++g) {
+      const int first = (int)((int64_t)tiles * g / ng), last = (int)((int64_t)tiles * (g + 1) / ng);
+      groups.push_back({i, first, last - first});
+    }
   }
-  std::vector<std::vector<int>> qs(GROUPED_QUEUES);
-  for(int i : order) qs[q_of[i]].push_back(i);
+  auto work = [&](const TileGroup& g) { return tile_ms(P[g.problem].config, P[g.problem].K) * g.count; };
+  std::vector<int> by_work(groups.size());
+  std::iota(by_work.begin(), by_work.end(), 0);
+  std::stable_sort(by_work.begin(), by_work.end(), [&](int a, int b) { return work(groups[a]) > work(groups[b]); });
+  std::vector<double> load(GROUPED_QUEUES, 0.0);
+  std::vector<int> q_of(groups.size(), 0);
+  for(int gi : by_work) {
+    const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    q_of[gi] = q;
+    load[q] += work(groups[gi]);
+  }
+  std::vector<std::vector<TileGroup>> qs(GROUPED_QUEUES);
+  for(size_t gi = 0; gi < groups.size(); ++gi) qs[q_of[gi]].push_back(groups[gi]);  // groups are in `order`
   return qs;
 }
 
@@ -406,7 +444,11 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
   pl.per_level = (pol.per_level == 1 && max_level > 0) || pl.pipelined;
   std::vector<int> cfg(count);
   if(pl.pipelined) {
-    for(int i = 0; i < count; ++i) cfg[i] = problem_config(pol, prec, problems[i].m, problems[i].n);
+    // Problems with producers sit on the critical path of the segments (a
+    // chain's later levels): they take the small tile, so each of their
+    // segments is short and the independent problems fill it.
+    for(int i = 0; i < count; ++i)
+      cfg[i] = has_prod[i] ? pl.small : problem_config(pol, prec, problems[i].m, problems[i].n);
   } else {
     cfg = assign_configs(pol, prec, problems, count, level, cus);
   }
@@ -477,7 +519,11 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
     P.packed_a = P.packed_b = nullptr;
     P.pack_a_ctr = P.pack_b_ctr = 0;
     P.prepacked = 0;
-    if(packed && P.tiles_m * P.tiles_n > 0) {
+    P.direct = packed && pol.direct && P.tiles_m * P.tiles_n > 0 && q.k % 16 == 0 &&
+               direct_operand_ok(q.A, q.lda, P.M, P.a_kmajor) && direct_operand_ok(q.B, q.ldb, P.N, P.b_kmajor);
+    if(P.direct) {
+      P.prepacked = 1;  // nothing to pack: the tiles read A and B themselves
+    } else if(packed && P.tiles_m * P.tiles_n > 0) {
       // load_chunk8 takes 128-bit loads along k only
       P.vec_a = P.a_kmajor && P.vec_a;
       P.vec_b = P.b_kmajor && P.vec_b;
@@ -567,14 +613,16 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
       std::vector<GItem> pre_items;
       for(int b = 0; b < blocks; b += PREPACK_BLOCKS)
         pre_items.push_back({ITEM_PREPACK, 0, b, std::min(PREPACK_BLOCKS, blocks - b)});
-      const auto qs = queue_problems(at, pl.problems);
+      const auto qs = queue_tiles(at, pl.problems);
       pl.qbeg.push_back(0);
       size_t next_pre = 0;
       for(int q = 0; q < GROUPED_QUEUES; ++q) {
         std::vector<GItem> tiles;
-        for(int i : qs[q])
-          for(int tm = 0; tm < pl.problems[i].tiles_m; ++tm)
-            for(int tn = 0; tn < pl.problems[i].tiles_n; ++tn) tiles.push_back({ITEM_TILE, i, tm, tn});
+        for(const TileGroup& tg : qs[q]) {
+          const int tn_count = pl.problems[tg.problem].tiles_n;
+          for(int t = tg.first; t < tg.first + tg.count; ++t)
+            tiles.push_back({ITEM_TILE, tg.problem, t / tn_count, t % tn_count});
+        }
         // this queue's share of the pre-pack items, spread evenly over its tiles
         const size_t share = (pre_items.size() * (q + 1)) / GROUPED_QUEUES - (pre_items.size() * q) / GROUPED_QUEUES;
         const size_t nt = tiles.size();

@@ -142,8 +142,15 @@ __device__ __forceinline__ void run_tile(const BfTileArgs& t, int tm, int tn, u3
 }
 
 template <int NHL, int C>
+__device__ __forceinline__ void run_direct(const DirectTileArgs& t, int tm, int tn, u32x4* lds) {
+  bf16_tile_direct<Geo<C>::BM, Geo<C>::BN, Geo<C>::WM, Geo<C>::WN, NHL, PIPE_NB>(t, tm, tn, lds);
+}
+
+template <int NHL, int C>
 constexpr int bf16_smem_chunks() {
-  return BfPipeCfg<Geo<C>::BM, Geo<C>::BN, Geo<C>::WM, Geo<C>::WN, NHL, pipe_kb<NHL>(), PIPE_NB>::SMEM_CHUNKS;
+  constexpr int p = BfPipeCfg<Geo<C>::BM, Geo<C>::BN, Geo<C>::WM, Geo<C>::WN, NHL, pipe_kb<NHL>(), PIPE_NB>::SMEM_CHUNKS;
+  constexpr int d = BfDirectCfg<Geo<C>::BM, Geo<C>::BN, Geo<C>::WM, Geo<C>::WN, PIPE_NB>::SMEM_CHUNKS;
+  return p > d ? p : d;
 }
 
 template <int NHL, int BIG, int SMALL>
@@ -174,6 +181,23 @@ __global__ __launch_bounds__(geo_threads<BIG>()) void grouped_bf16x_kernel(const
       pack_item<NHL, NT>(P, is_a, item.x, item.y);
       const int panel = item.x * 32 / (is_a ? bm : bn);
       signal(&g.pack_ctr[(is_a ? P.pack_a_ctr : P.pack_b_ctr) + panel]);
+      continue;
+    }
+    if(P.direct) {
+      // in-kernel mode: the producers' results are read straight from C
+      if(P.ndeps > 0 && !wg_wait(&ready, [&] { return poll_producers(P, g); })) continue;
+      DirectTileArgs d;
+      d.a = DirectOp{P.A, P.lda, P.M, P.a_kmajor};
+      d.b = DirectOp{P.B, P.ldb, P.N, P.b_kmajor};
+      d.K = P.K;
+      d.ep = Epilogue{P.C, P.ldc, P.alpha, P.beta, P.M, P.N, TILES_FULL, 0};
+      if constexpr(BIG == SMALL) {
+        run_direct<NHL, BIG>(d, item.x, item.y, lds);
+      } else {
+        if(P.config == BIG) run_direct<NHL, BIG>(d, item.x, item.y, lds);
+        else run_direct<NHL, SMALL>(d, item.x, item.y, lds);
+      }
+      if(P.consumed) signal(&g.done[item.problem]);
       continue;
     }
     const int nblk_a = (P.M + 31) / 32, nblk_b = (P.N + 31) / 32;

@@ -511,4 +511,168 @@ __device__ __forceinline__ void bf16_tile_pipe(const BfTileArgs& t, int tm, int 
   });
 }
 
+// ---------------------------------------------------------------------------
+// Direct tile: the FP32 operands go from global memory straight into LDS
+// (global_load_lds_dwordx4, no registers) and each wave splits its own
+// fragments into hi / lo BF16 right before its MFMAs -- no packing pass and
+// no packed copy in HBM.  The split is split_chunk's, applied to the same 8
+// consecutive k of the same row, and the MFMA sequence is bf16_tile_pipe's,
+// so the result is bit-identical to the packed path.
+//
+// One LDS stage holds 16 k of both panels; a 32-row (or 32-column) block of
+// an operand is 128 16-byte chunks laid out so that one DMA instruction of a
+// wave fills 64 consecutive chunks:
+//   k-major source (element (r, k) at p[r*ld + k]): chunk h*64 + l of a
+//     block holds row l%32, k = 8*(l/32) + 4h .. +3, so a lane's fragment
+//     (8 consecutive k of its row) is chunks l and 64 + l;
+//   row-major source (element (r, k) at p[k*ld + r]): the block is a 16 x 32
+//     (k x r) float array, row k at floats k*32 .. k*32+31, filled 4
+//     consecutive r per chunk, and a lane reads its 8 k one float each.
+// Rows past the operand read its last row (or last 4 columns) again; they
+// only feed outputs the epilogue does not store.  Requirements (checked by
+// the host): K a multiple of 16; 16-byte aligned operands with ld % 4 == 0;
+// for a row-major source also rows % 4 == 0.
+struct DirectOp {
+  const float* p;
+  int64_t ld;
+  int rows;    // logical rows of the operand's role (M for A, N for B)
+  int kmajor;
+};
+struct DirectTileArgs {
+  DirectOp a, b;
+  int K;
+  Epilogue ep;
+};
+
+template <int BM, int BN, int WM, int WN, int NB>
+struct BfDirectCfg {
+  static constexpr int NWM = BM / WM, NWN = BN / WN, NW = NWM * NWN, NT = NW * 64;
+  static constexpr int FM = WM / 32, FN = WN / 32;
+  static constexpr int CH_A = BM / 32 * 128, CH_B = BN / 32 * 128;  // 16-byte chunks per stage
+  static constexpr int PER_A = CH_A / NT, PER_B = CH_B / NT;
+  static constexpr int SMEM_CHUNKS = NB * (CH_A + CH_B);
+  static_assert(CH_A % NT == 0 && CH_B % NT == 0, "stage must split evenly");
+  static_assert(NB >= 2 && (NB - 2) * (PER_A + PER_B) < 64, "in-flight DMA count fits vmcnt");
+};
+
+// Source of DMA group g (64 chunks: block g / 2, half g % 2) for lane l of a
+// tile whose operand rows start at row0, at k = 0.
+__device__ __forceinline__ const float* direct_src(const DirectOp& o, int row0, int g, int l) {
+  const int blk = g >> 1, h = g & 1;
+  if(o.kmajor) {
+    const int r = min(row0 + blk * 32 + (l & 31), o.rows - 1);
+    return o.p + (int64_t)r * o.ld + 8 * (l >> 5) + 4 * h;
+  }
+  const int c = min(row0 + blk * 32 + (l & 7) * 4, o.rows - 4);
+  return o.p + (int64_t)(h * 8 + (l >> 3)) * o.ld + c;
+}
+
+// Lane l's fragment of block blk (8 consecutive k of its row) from a stage.
+__device__ __forceinline__ void direct_frag(const u32x4* st, int kmajor, int blk, int l, float (&x)[8]) {
+  if(kmajor) {
+    const floatx4 u = __builtin_bit_cast(floatx4, st[blk * 128 + l]);
+    const floatx4 v = __builtin_bit_cast(floatx4, st[blk * 128 + 64 + l]);
+#pragma unroll
+    for(int e = 0; e < 4; ++e) {
+      x[e] = u[e];
+      x[4 + e] = v[e];
+    }
+  } else {
+    const float* f = reinterpret_cast<const float*>(st + blk * 128);
+#pragma unroll
+    for(int e = 0; e < 8; ++e) x[e] = f[(8 * (l >> 5) + e) * 32 + (l & 31)];
+  }
+}
+
+template <int BM, int BN, int WM, int WN, int NHL, int NB>
+__device__ __forceinline__ void bf16_tile_direct(const DirectTileArgs& t, int tm, int tn, u32x4* lds) {
+  using Cfg = BfDirectCfg<BM, BN, WM, WN, NB>;
+  constexpr int NT = Cfg::NT, NW = Cfg::NW, FM = Cfg::FM, FN = Cfg::FN;
+  constexpr int SA = Cfg::CH_A, SB = Cfg::CH_B;
+  constexpr int PER = Cfg::PER_A + Cfg::PER_B;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm = wid / Cfg::NWN, wn = wid % Cfg::NWN;
+
+  floatx16 acc[FM][FN];
+  zero_acc(acc);
+
+  const int steps = t.K / 16;
+  const int wave_base = __builtin_amdgcn_readfirstlane(tid - lane);
+  // per DMA instruction: its source at k = 0; a stage advances k by 16
+  const float* src_a[Cfg::PER_A];
+  const float* src_b[Cfg::PER_B];
+#pragma unroll
+  for(int i = 0; i < Cfg::PER_A; ++i) src_a[i] = direct_src(t.a, tm * BM, wid + i * NW, lane);
+#pragma unroll
+  for(int i = 0; i < Cfg::PER_B; ++i) src_b[i] = direct_src(t.b, tn * BN, wid + i * NW, lane);
+  const int64_t step_a = t.a.kmajor ? 16 : 16 * t.a.ld, step_b = t.b.kmajor ? 16 : 16 * t.b.ld;
+  auto issue = [&](int s) __attribute__((always_inline)) {
+    u32x4* sa = lds + (s % NB) * (SA + SB);
+    u32x4* sb = sa + SA;
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_A; ++i)
+      lds_dma16(reinterpret_cast<const u32x4*>(src_a[i] + s * step_a), sa + wave_base + i * NT);
+#pragma unroll
+    for(int i = 0; i < Cfg::PER_B; ++i)
+      lds_dma16(reinterpret_cast<const u32x4*>(src_b[i] + s * step_b), sb + wave_base + i * NT);
+  };
+  auto contract = [&](const u32x4* sa, const u32x4* sb) __attribute__((always_inline)) {
+    u32x4 ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+    for(int i = 0; i < FM; ++i) {
+      float x[8];
+      direct_frag(sa, t.a.kmajor, wm * FM + i, lane, x);
+      split_chunk<NHL>(x, ah[i], al[i]);
+    }
+#pragma unroll
+    for(int j = 0; j < FN; ++j) {
+      float x[8];
+      direct_frag(sb, t.b.kmajor, wn * FN + j, lane, x);
+      split_chunk<NHL>(x, bh[j], bl[j]);
+    }
+    if(NHL == 2) {
+#pragma unroll
+      for(int i = 0; i < FM; ++i)
+#pragma unroll
+        for(int j = 0; j < FN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(al[i]), as_bf16x8(bh[j]), acc[i][j], 0, 0, 0);
+#pragma unroll
+      for(int i = 0; i < FM; ++i)
+#pragma unroll
+        for(int j = 0; j < FN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(ah[i]), as_bf16x8(bl[j]), acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for(int i = 0; i < FM; ++i)
+#pragma unroll
+      for(int j = 0; j < FN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(ah[i]), as_bf16x8(bh[j]), acc[i][j], 0, 0, 0);
+  };
+
+#pragma unroll
+  for(int p = 0; p < NB - 1; ++p)
+    if(p < steps) issue(p);
+  for(int s = 0; s < steps; ++s) {
+    const int after = steps - 1 - s;
+    if(after >= NB - 2) vm_wait<(NB - 2) * PER>();
+    else if(NB > 3 && after == 1) vm_wait<PER>();
+    else lds_dma_wait();
+    __syncthreads();
+    if(s + NB - 1 < steps) issue(s + NB - 1);
+    const u32x4* sa = lds + (s % NB) * (SA + SB);
+    contract(sa, sa + SA);
+  }
+
+  static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
+    constexpr int i = decltype(ij)::value / FN, j = decltype(ij)::value % FN;
+    store_block32(t.ep, acc[i][j], tm * BM + wm * WM + i * 32, tn * BN + wn * WN + j * 32, lane);
+  });
+}
+
+// Host: whether an operand can be read by the direct tile.
+inline bool direct_operand_ok(const float* p, int64_t ld, int rows, bool kmajor) {
+  if((reinterpret_cast<uintptr_t>(p) & 15u) != 0 || ld % 4 != 0) return false;
+  return kmajor || rows % 4 == 0;
+}
+
 }  // namespace mfx

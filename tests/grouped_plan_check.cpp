@@ -208,6 +208,12 @@ static void check_plan(const Case& cs, mfx_precision prec, int policy) {
   }
   for(int i = 0; i < count; ++i) {
     const GProblem& P = pl.problems[i];
+    if(P.direct) {
+      const auto& q = cs.probs[i];
+      const bool ok = q.k % 16 == 0 && q.lda % 4 == 0 && q.ldb % 4 == 0 && (P.a_kmajor || P.M % 4 == 0) &&
+                      (P.b_kmajor || P.N % 4 == 0) && P.kbs == 0;
+      CHECK(ok, "problem %d runs direct tiles it cannot", i);
+    }
     if(P.tiles_m * P.tiles_n == 0 || P.kbs == 0) {
       const bool none = !packs.count(std::make_pair(i, 0)) && !packs.count(std::make_pair(i, 1));
       CHECK(none, "problem %d without work is packed", i);

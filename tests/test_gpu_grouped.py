@@ -284,3 +284,52 @@ def test_chain_every_tile_policy(handle, mfx, prec):
             assert rel(out, want) < TOL[prec] * 3, pol
     finally:
         handle.force_grouped_policy(-1)
+
+
+@pytest.mark.parametrize("policy", [25, 29, 33])
+@pytest.mark.parametrize("prec", ["split-bf16", "bf16"])
+def test_direct_tiles_edges(handle, mfx, prec, policy):
+    """Policies whose tiles read FP32 operands directly (no packing): ragged
+    M / N (rows past the operand read its last row again), K = 16 and K = 0,
+    padded leading dimensions, every transpose, plus problems the direct
+    tile must not take (K % 16 != 0, a row-major operand whose row count is
+    not a multiple of 4, a base address off 16 bytes), which fall back to
+    packing -- all against FP64, and a consumer reading a producer's C."""
+    shapes = [(300, 260, 64, False, False), (36, 44, 16, True, False), (260, 100, 48, False, True),
+              (100, 36, 32, True, True), (64, 72, 0, False, False), (70, 66, 40, False, False),
+              (33, 40, 24, False, False), (40, 35, 48, True, True), (37, 40, 32, True, False)]
+    keep, probs = [], []
+    for i, (m, n, k, ta, tb) in enumerate(shapes):
+        pad = 4 * (i % 3)
+        A = torch.empty(((k, m + pad) if ta else (m, k + pad)), device=DEV)[:, : (m if ta else k)]
+        B = torch.empty(((n, k + pad) if tb else (k, n + pad)), device=DEV)[:, : (k if tb else n)]
+        C = torch.empty((m, n), device=DEV)
+        for j, x in enumerate((A, B, C)):
+            if x.numel():
+                handle.fill_uniform(x, 17, 3 * i + j)
+        keep.append((A, B, C, ta, tb, C.clone()))
+        probs.append(mfx.problem(A, B, C, alpha=0.75, beta=1.5, transA=ta, transB=tb))
+    # a view that starts one float into its storage: not 16-byte aligned
+    base = torch.empty((129, 97), device=DEV)
+    handle.fill_uniform(base, 17, 90)
+    A = base.view(-1)[1:1 + 128 * 96].view(128, 96)
+    B = torch.empty((96, 64), device=DEV)
+    handle.fill_uniform(B, 17, 91)
+    C = torch.empty((128, 64), device=DEV)
+    handle.fill_uniform(C, 17, 92)
+    keep.append((A, B, C, False, False, C.clone()))
+    probs.append(mfx.problem(A, B, C, alpha=0.75, beta=1.5))
+    # consumer of problem 0's output: E = C0 (300 x 260) * F (260 x 48)
+    F = torch.empty((260, 48), device=DEV)
+    handle.fill_uniform(F, 17, 93)
+    E = torch.empty((300, 48), device=DEV)
+    probs.append(mfx.problem(keep[0][2], F, E))
+    try:
+        handle.force_grouped_policy(policy)
+        handle.grouped(probs, deps=[(len(probs) - 1, 0)], prec=prec)
+        handle.grouped_status()
+    finally:
+        handle.force_grouped_policy(-1)
+    for A, B, C, ta, tb, c0 in keep:
+        assert rel(C, fp64(A, B, c0, 0.75, 1.5, ta, tb)) < TOL[prec], (tuple(C.shape), ta, tb)
+    assert rel(E, keep[0][2].double() @ F.double()) < TOL[prec]
