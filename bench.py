@@ -205,6 +205,19 @@ def _time_ms(torch, fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
+def _kernel_ms(torch, h, fn, reps):
+    """Average duration (ms) of each kernel `fn` launches, from the library's
+    hipEvent pair around every launch, over `reps` calls (a separate pass from
+    the end-to-end timing, which the event records would perturb)."""
+    fn()
+    torch.cuda.synchronize()
+    h.timing_begin()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return {k: round(v[1] / v[0], 4) for k, v in h.timing_end().items()}
+
+
 def measure_configs(torch, mfx, h, reps=3):
     """The other workloads at their full sizes, end to end per call (packing
     included), with the relative Frobenius error against the GPU FP64 product:
@@ -219,8 +232,10 @@ def measure_configs(torch, mfx, h, reps=3):
         h.fill_uniform(x, 1, mid)
         return x
 
-    def rec(name, work, ms, err):
+    def rec(name, work, ms, err, fn=None):
         out[name] = {"ms": round(ms, 4), "tflops": round(work / (ms / 1e3) / 1e12, 1), "rel_err": err}
+        if fn is not None:
+            out[name]["kernel_avg_ms"] = _kernel_ms(torch, h, fn, reps)
         log(f"configs: {name}: {out[name]}")
 
     # gemm N=1024, FP32 (alpha 1.5, beta 1.2)
@@ -229,10 +244,11 @@ def measure_configs(torch, mfx, h, reps=3):
     R = C0.double()
     h.gemm_f64(A.double(), B.double(), R, alpha=1.5, beta=1.2)
     C = C0.clone()
-    ms = _time_ms(torch, lambda: h.gemm(A, B, C, alpha=1.5, beta=1.2, prec="fp32"), 20)
+    g1024 = lambda: h.gemm(A, B, C, alpha=1.5, beta=1.2, prec="fp32")
+    ms = _time_ms(torch, g1024, 20)
     C = C0.clone()
     h.gemm(A, B, C, alpha=1.5, beta=1.2, prec="fp32")
-    rec("gemm_1024_fp32", 2.0 * n ** 3, ms, h.rel_frobenius(C, R))
+    rec("gemm_1024_fp32", 2.0 * n ** 3, ms, h.rel_frobenius(C, R), g1024)
 
     # 2mm D = 1.5 (A B) C + 1.2 D and 3mm G = (A B)(C D), 4096^2
     n = 4096
@@ -270,9 +286,9 @@ def measure_configs(torch, mfx, h, reps=3):
             ms = _time_ms(torch, mm2, reps)  # D accumulates over the timed calls; re-run once for the error
             D.copy_(D0)
             mm2()
-            rec(f"2mm_4096_{tag}", 4.0 * n ** 3, ms, h.rel_frobenius(D, R2))
+            rec(f"2mm_4096_{tag}", 4.0 * n ** 3, ms, h.rel_frobenius(D, R2), mm2)
             ms = _time_ms(torch, mm3, reps)
-            rec(f"3mm_4096_{tag}", 6.0 * n ** 3, ms, h.rel_frobenius(G, G64))
+            rec(f"3mm_4096_{tag}", 6.0 * n ** 3, ms, h.rel_frobenius(G, G64), mm3)
     del A, B, Cm, D0, T, D, E, F, G, R2, G64
     torch.cuda.empty_cache()
 
@@ -292,15 +308,16 @@ def measure_configs(torch, mfx, h, reps=3):
     C = torch.empty_like(C0)
     for prec in ("fp32", "split-bf16", "bf16"):
         tag = prec.replace("-", "")
-        ms = _time_ms(torch, lambda: h.syrk(A, C, alpha=1.5, beta=1.2, prec=prec, uplo=mfx.LOWER, mirror=False), reps)
+        sy = lambda: h.syrk(A, C, alpha=1.5, beta=1.2, prec=prec, uplo=mfx.LOWER, mirror=False)
+        ms = _time_ms(torch, sy, reps)
         C.copy_(C0)
         h.syrk(A, C, alpha=1.5, beta=1.2, prec=prec, uplo=mfx.LOWER, mirror=False)
-        rec(f"syrk_8192_lower_{tag}", float(n) * (n + 1) * n, ms, h.rel_frobenius(C, R1, uplo=mfx.LOWER))
-        ms = _time_ms(torch, lambda: h.syr2k(A, B, C, alpha=1.5, beta=1.2, prec=prec, uplo=mfx.LOWER, mirror=False),
-                      reps)
+        rec(f"syrk_8192_lower_{tag}", float(n) * (n + 1) * n, ms, h.rel_frobenius(C, R1, uplo=mfx.LOWER), sy)
+        sy2 = lambda: h.syr2k(A, B, C, alpha=1.5, beta=1.2, prec=prec, uplo=mfx.LOWER, mirror=False)
+        ms = _time_ms(torch, sy2, reps)
         C.copy_(C0)
         h.syr2k(A, B, C, alpha=1.5, beta=1.2, prec=prec, uplo=mfx.LOWER, mirror=False)
-        rec(f"syr2k_8192_lower_{tag}", 2.0 * n * (n + 1) * n, ms, h.rel_frobenius(C, R2, uplo=mfx.LOWER))
+        rec(f"syr2k_8192_lower_{tag}", 2.0 * n * (n + 1) * n, ms, h.rel_frobenius(C, R2, uplo=mfx.LOWER), sy2)
     del A, B, C, C0, R1, R2
     torch.cuda.empty_cache()
 
@@ -313,8 +330,9 @@ def measure_configs(torch, mfx, h, reps=3):
     while len(level) > 1:
         level = [level[i] @ level[i + 1] for i in range(0, len(level), 2)]
     for prec in ("fp32", "split-bf16"):
-        ms = _time_ms(torch, lambda: h.chain(mats, scratch, res, prec=prec), reps)
-        rec(f"matmulchain_8x2048_{prec.replace('-', '')}", 7 * 2.0 * n ** 3, ms, h.rel_frobenius(res, level[0]))
+        chn = lambda: h.chain(mats, scratch, res, prec=prec)
+        ms = _time_ms(torch, chn, reps)
+        rec(f"matmulchain_8x2048_{prec.replace('-', '')}", 7 * 2.0 * n ** 3, ms, h.rel_frobenius(res, level[0]), chn)
     del mats, scratch, res, level
     torch.cuda.empty_cache()
 
@@ -327,9 +345,10 @@ def measure_configs(torch, mfx, h, reps=3):
     R = A[r0:r0 + blk].double() @ B[:, c0:c0 + blk].double()
     C = torch.empty((n, n), device=dev)
     for prec in ("fp8-block", "fp8-block2"):
-        ms = _time_ms(torch, lambda: h.gemm(A, B, C, prec=prec), reps)
+        g8 = lambda: h.gemm(A, B, C, prec=prec)
+        ms = _time_ms(torch, g8, reps)
         err = h.rel_frobenius(C[r0:r0 + blk, c0:c0 + blk], R)
-        rec(f"gemm_16384_{prec.replace('-', '')}", 2.0 * n ** 3, ms, err)
+        rec(f"gemm_16384_{prec.replace('-', '')}", 2.0 * n ** 3, ms, err, g8)
     del A, B, C, R
     torch.cuda.empty_cache()
     return out
