@@ -60,7 +60,9 @@ mfx_status mfx_create(mfx_handle* handle, int device);
 mfx_status mfx_destroy(mfx_handle handle);
 /* Pre-allocates `bytes` of workspace so later calls (and graph capture) do not allocate. */
 mfx_status mfx_reserve_workspace(mfx_handle handle, size_t bytes);
-/* Workspace bytes a gemm of this shape and precision needs (0 for FP32). */
+/* Workspace bytes a gemm of this shape and precision needs under the
+ * automatic choices on the current device (FP32: split-K partial tiles of a
+ * small grid, at most 32 MiB, else 0). */
 size_t mfx_gemm_workspace_size(mfx_precision prec, int64_t m, int64_t n, int64_t k);
 /*
  * Host-only pre-flight check (no handle, no device): the status mfx_gemm
@@ -98,9 +100,9 @@ mfx_status mfx_gemm(mfx_handle handle, mfx_precision prec, mfx_op transA, mfx_op
 
 /*
  * mfx_gemm without any allocation: the same kernels and results as mfx_gemm
- * (FP32, SPLIT_BF16 or BF16), with the packed operands in a fixed library
- * array of MFX_NOALLOC_SCRATCH_BYTES per device instead of the handle's
- * workspace.  It calls no hipMalloc / hipFree or other allocating routine,
+ * (FP32, SPLIT_BF16 or BF16), with the packed operands (or FP32 split-K
+ * partial tiles, at most 32 MiB) in a fixed library array of
+ * MFX_NOALLOC_SCRATCH_BYTES per device instead of the handle's workspace.  It calls no hipMalloc / hipFree or other allocating routine,
  * so code under graph capture, tight serving loops and harnesses that
  * forbid allocation can call it.  A request whose packed operands exceed the
  * array (4 (m + n) k bytes for SPLIT_BF16, half that for BF16, about) returns
@@ -348,6 +350,13 @@ mfx_status mfx_config_info(mfx_precision prec, int32_t config, int32_t* bm, int3
     int32_t* bk, int32_t* waves);
 /* Forces `config` (or -1 for the tuned / heuristic choice) for later calls of `prec`. */
 mfx_status mfx_force_config(mfx_handle handle, mfx_precision prec, int32_t config);
+/* FP32 split-K override for later single-product calls: -1 automatic (the
+ * default: k slices when the tile grid leaves the CUs short of work), 1 never,
+ * >= 2 that many k slices (fewer when K or the 32 MiB partial-tile bound does
+ * not allow them; configurations with tiles above 128x128 never split).
+ * Split results are summed in slice order, so they are reproducible, but not
+ * bit-identical to the unsplit kernel's.  No counterpart in the reference. */
+mfx_status mfx_force_splits(mfx_handle handle, int32_t splits);
 
 #ifdef __cplusplus
 }

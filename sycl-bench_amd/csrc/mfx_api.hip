@@ -147,12 +147,12 @@ void replace_tuned(const std::map<std::string, int>& t) {
 // large problems (the best configuration's time over this one's, measured on
 // MI355X at M = N = K = 8192).
 double config_rate(mfx_precision p, int c) {
-  static const double f32[] = {0.886, 0.868, 0.817, 0.859, 1.0, 0.97, 0.913};
+  static const double f32[] = {0.886, 0.868, 0.817, 0.859, 1.0, 0.97, 0.913, 0.81};
   static const double split[] = {0.894, 0.846, 0.645, 0.70, 1.0, 0.80, 0.854};
   static const double bf16[] = {0.844, 0.730, 0.508, 0.662, 1.0, 0.724, 0.814};
   static const double fp8[] = {0.866, 0.69, 1.0};
   const double* t = p == MFX_PREC_FP32 ? f32 : p == MFX_PREC_SPLIT_BF16 ? split : p == MFX_PREC_BF16 ? bf16 : fp8;
-  const int n = p == MFX_PREC_FP32 || p == MFX_PREC_SPLIT_BF16 || p == MFX_PREC_BF16 ? 7 : 3;
+  const int n = p == MFX_PREC_FP32 ? 8 : p == MFX_PREC_SPLIT_BF16 || p == MFX_PREC_BF16 ? 7 : 3;
   return c >= 0 && c < n ? t[c] : 0.5;
 }
 
@@ -209,8 +209,17 @@ int least_time_config(Handle* h, mfx_precision p, bool square_only, Tiles&& tile
   return best < 0 ? 0 : best;
 }
 
+// FP32 configuration with sixteen waves on a 64x64 tile (four k-groups).
+constexpr int kF32KGroupConfig = 7;
+
 int heuristic_config(Handle* h, mfx_precision p, int64_t m, int64_t n, int64_t k) {
   (void)k;
+  // FP32 grids of at most one 64x64 tile per CU: the k-grouped tile (one
+  // sixteen-wave workgroup per CU hides load latency that four-wave tiles
+  // leave exposed at this size; split-K then fills the CUs if tiles are
+  // few).  Measured on MI355X: 1024^3 in 27-30 us against 32-36 us.
+  const int cus = h->cu_count > 0 ? h->cu_count : 256;
+  if(p == MFX_PREC_FP32 && ceil_div(m, 64) * ceil_div(n, 64) <= cus) return kF32KGroupConfig;
   return least_time_config(h, p, false, [&](int bm, int bn) { return ceil_div(m, bm) * ceil_div(n, bn); });
 }
 
@@ -409,10 +418,25 @@ size_t mfx_gemm_workspace_size(mfx_precision prec, int64_t m, int64_t n, int64_t
   op.K = (int)k;
   op.nprod = 1;
   size_t best = 0;
+  // FP32 needs workspace only for split-K partial tiles, whose count depends
+  // on the CUs of the current device.
+  int cus = 0;
+  if(prec == MFX_PREC_FP32) {
+    int dev = 0;
+    if(hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+  }
   // The largest need over all configurations, so any tuned choice fits.
   for(int c = 0; c < config_count(prec); ++c) {
     size_t b = 0;
-    if(prec == MFX_PREC_SPLIT_BF16 || prec == MFX_PREC_BF16)
+    if(prec == MFX_PREC_FP32) {
+      int bm, bn, bk, w;
+      config_info(prec, c, &bm, &bn, &bk, &w);
+      const int64_t tiles = ceil_div(m, bm) * ceil_div(n, bn);
+      const int splits = f32_splits(c, tiles, k, cus);
+      if(splits > 1) b = f32_splitk_workspace(c, tiles, splits);
+    } else if(prec == MFX_PREC_SPLIT_BF16 || prec == MFX_PREC_BF16)
       b = dense_bf16x_workspace(op, prec == MFX_PREC_SPLIT_BF16, c);
     else if(prec == MFX_PREC_FP8_BLOCK || prec == MFX_PREC_FP8_BLOCK2)
       b = dense_fp8_workspace(op, prec == MFX_PREC_FP8_BLOCK2, c);
@@ -625,6 +649,13 @@ mfx_status mfx_force_config(mfx_handle handle, mfx_precision prec, int32_t confi
   if(!valid_prec(prec)) return invalid("unknown precision");
   if(config < -1 || config >= config_count(prec)) return invalid("configuration out of range");
   h->forced_config[prec] = config;
+  return MFX_SUCCESS;
+}
+
+mfx_status mfx_force_splits(mfx_handle handle, int32_t splits) {
+  HANDLE_OR_FAIL(handle);
+  if(splits < -1 || splits == 0 || splits > 64) return invalid("splits must be -1, or 1 to 64");
+  h->forced_splits = splits;
   return MFX_SUCCESS;
 }
 

@@ -67,6 +67,7 @@ struct Handle {
   size_t staging_bytes = 0;
   hipEvent_t staging_ready = nullptr;
   int* error_flag = nullptr;
+  int forced_splits = -1;  // FP32 split-K: -1 automatic, 1 never, >= 2 that many slices
 };
 
 // Looks up the device's fixed no-allocation scratch array (mfx_create calls
@@ -331,6 +332,14 @@ struct DenseOp {
 };
 
 mfx_status launch_dense_f32(Handle* h, const DenseOp& op, int config, hipStream_t s);
+// The same with split-K partial tiles in `ws` (ws_bytes long) instead of the
+// handle's workspace; fails if a split this call makes needs more.
+mfx_status launch_dense_f32_ws(Handle* h, const DenseOp& op, int config, void* ws, size_t ws_bytes, hipStream_t s);
+// FP32 split-K pieces (gemm_f32.hip): k slices per tile for a configuration
+// (1: none), the partial buffer's size, and the launch on given buffers.
+int f32_splits(int config, int64_t tiles, int64_t K, int cus);
+size_t f32_splitk_workspace(int config, int64_t tiles, int splits);
+mfx_status launch_f32_splitk(Handle* h, const DenseOp& op, int config, int splits, float* partial, hipStream_t s);
 mfx_status launch_dense_bf16x(Handle* h, const DenseOp& op, bool split, int config, hipStream_t s);
 // The same on a caller-provided workspace of dense_bf16x_workspace() bytes.
 mfx_status launch_dense_bf16x_ws(Handle* h, const DenseOp& op, bool split, int config, void* ws, hipStream_t s);

@@ -89,7 +89,11 @@ mfx_status mfx_gemm_noalloc(mfx_handle handle, mfx_precision prec, mfx_op transA
   h->timing = false;
   mfx_status st;
   if(prec == MFX_PREC_FP32) {
-    st = dispatch_dense(h, prec, op, cfg, s);  // reads A and B directly: no scratch
+    // A and B are read directly; a split-K call keeps its partial tiles
+    // (at most 32 MiB) in the fixed array
+    void* ws = scratch_of(h->device);
+    if(!ws) st = invalid("mfx_gemm_noalloc: no scratch address for this device (handle not made by mfx_create)");
+    else st = launch_dense_f32_ws(h, op, cfg, ws, MFX_NOALLOC_SCRATCH_BYTES, s);
   } else {
     const bool split = prec == MFX_PREC_SPLIT_BF16;
     const size_t need = dense_bf16x_workspace(op, split, cfg);

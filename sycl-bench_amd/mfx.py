@@ -91,6 +91,7 @@ SIGNATURES = {
     "mfx_config_info": (_ST, [_ST, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I32),
                               ctypes.POINTER(_I32)]),
     "mfx_force_config": (_ST, [_V, _ST, _I32]),
+    "mfx_force_splits": (_ST, [_V, _I32]),
 }
 
 _lib = None
@@ -268,6 +269,10 @@ class Handle:
 
     def force_config(self, prec, config):
         _check(self.lib.mfx_force_config(self.h, prec_id(prec), config), "mfx_force_config")
+
+    def force_splits(self, splits):
+        """FP32 split-K override: -1 automatic, 1 never, >= 2 that many k slices."""
+        _check(self.lib.mfx_force_splits(self.h, splits), "mfx_force_splits")
 
     def force_grouped_policy(self, policy):
         _check(self.lib.mfx_force_grouped_policy(self.h, policy), "mfx_force_grouped_policy")

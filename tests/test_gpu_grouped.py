@@ -65,6 +65,7 @@ def test_grouped_bitwise_equals_single_launch(handle, mfx, prec, config):
     fill(handle, keep, 6)
     C0 = [k[2].clone() for k in keep]
     handle.force_config(prec, config)
+    handle.force_splits(1)  # the grouped kernels run each tile whole: compare with unsplit single launches
     try:
         probs = [mfx.problem(A, B, C, alpha=0.5, beta=-1.0, transA=ta, transB=tb) for A, B, C, ta, tb, _ in keep]
         handle.grouped(probs, prec=prec)
@@ -75,6 +76,7 @@ def test_grouped_bitwise_equals_single_launch(handle, mfx, prec, config):
             assert torch.equal(C, single)
     finally:
         handle.force_config(prec, -1)
+        handle.force_splits(-1)
 
 
 def chain_fp64(mats):
@@ -211,6 +213,7 @@ def test_every_tile_policy(handle, mfx, prec):
     and bit-equal to single-problem launches on the tile each problem got."""
     rng = np.random.default_rng(11)
     keep = rand_problems(mfx, rng, 24, lo=16, hi=700)
+    handle.force_splits(1)  # grouped tiles run whole: compare with unsplit single launches
     try:
         for pol in range(mfx.grouped_policy_count()):
             big, small, bound, per_level = mfx.grouped_policy_info(pol)
@@ -240,6 +243,7 @@ def test_every_tile_policy(handle, mfx, prec):
     finally:
         handle.force_grouped_policy(-1)
         handle.force_config(prec, -1)
+        handle.force_splits(-1)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "split-bf16", "bf16"])
