@@ -69,6 +69,8 @@ constexpr GroupedPolicy kPolicies[] = {
     {4, 4, 0.0, 2, 1},   // 31: pipelined segments
     {4, 6, 1.34, 2, 1},  // 32
     {4, 5, 1.34, 2, 1},  // 33
+    {4, 6, 0.0, 2, 1},   // 34: independent problems on the large tile, problems with
+    {4, 5, 0.0, 2, 1},   // 35:   producers (a chain's later levels) on the small one
 };
 constexpr int kPolicyCount = sizeof(kPolicies) / sizeof(kPolicies[0]);
 
