@@ -337,3 +337,28 @@ def test_direct_tiles_edges(handle, mfx, prec, policy):
     for A, B, C, ta, tb, c0 in keep:
         assert rel(C, fp64(A, B, c0, 0.75, 1.5, ta, tb)) < TOL[prec], (tuple(C.shape), ta, tb)
     assert rel(E, keep[0][2].double() @ F.double()) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "split-bf16", "bf16"])
+def test_k0_consumer_waits_for_its_producer(handle, mfx, prec):
+    """A K = 0 problem whose C is its producer's output (C = beta * C, no
+    product): under every policy it must run after the producer, so the
+    result is beta times the producer's product -- in-kernel waits included
+    (its tiles have nothing to pack, but still wait)."""
+    m, n, k = 300, 200, 96
+    A = torch.empty((m, k), device=DEV)
+    B = torch.empty((k, n), device=DEV)
+    handle.fill_uniform(A, 31, 0)
+    handle.fill_uniform(B, 31, 1)
+    want = 0.5 * (A.double() @ B.double())
+    try:
+        for pol in range(mfx.grouped_policy_count()):
+            X = torch.full((m, n), float("nan"), device=DEV)
+            probs = [mfx.problem(A, B, X, alpha=1.0, beta=0.0),
+                     mfx.problem(torch.zeros((m, 0), device=DEV), torch.zeros((0, n), device=DEV), X, beta=0.5)]
+            handle.force_grouped_policy(pol)
+            handle.grouped(probs, deps=[(1, 0)], prec=prec)
+            handle.grouped_status()
+            assert rel(X, want) < TOL[prec], pol
+    finally:
+        handle.force_grouped_policy(-1)
