@@ -154,6 +154,35 @@ def grouped_workload(torch, mfx, h, steps, warmup, seed=1):
     torch.cuda.synchronize()
     graph.status()
     ms = e0.elapsed_time(e1) / steps
+
+    def kernels_per_launch():
+        h.timing_begin()
+        h.grouped(probs, deps, prec="split-bf16")
+        rec = h.timing_end()
+        h.grouped_status()
+        return {name: int(n) for name, (n, _) in rec.items()}
+
+    kernels = kernels_per_launch()
+    # the same list as ONE persistent kernel with in-kernel dependency waits
+    # (policy 29: large tiles, direct operands), for comparison
+    single_policy = 29
+    h.force_grouped_policy(single_policy)
+    try:
+        single_kernels = kernels_per_launch()
+        g1 = h.graph_create(probs, deps, prec="split-bf16")
+        for _ in range(warmup):
+            g1.launch()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(steps):
+            g1.launch()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        g1.status()
+        single_ms = e0.elapsed_time(e1) / steps
+        g1.close()
+    finally:
+        h.force_grouped_policy(-1)
     # accuracy of the chain product and of a sample of the independent problems
     ref = chain[0].double()
     level = [c.double() for c in chain]
@@ -184,8 +213,15 @@ def grouped_workload(torch, mfx, h, steps, warmup, seed=1):
     t_single = s0.elapsed_time(s1)
     graph.close()
     return {"workload": "512 uneven problems (M,N,K log-uniform [64,2048], x16, seed 1) + 8-long chain of "
-                        "2048^2, split-BF16, one grouped launch (autotuned tile policy) replayed from a hipGraph",
+                        "2048^2, split-BF16, one mfx_gemm_grouped call (autotuned tile policy; its kernels per "
+                        "replay in kernels_per_replay) captured in a hipGraph and replayed",
             "flop": flop, "ms": round(ms, 4), "tflops": round(flop / (ms / 1e3) / 1e12, 2),
+            "kernels_per_replay": kernels,
+            "single_persistent_kernel": {"policy": single_policy, "ms": round(single_ms, 4),
+                                         "tflops": round(flop / (single_ms / 1e3) / 1e12, 2),
+                                         "kernels_per_replay": single_kernels,
+                                         "note": "the whole list as one persistent kernel with in-kernel "
+                                                 "dependency waits, same tiles, hipGraph replay"},
             "per_problem_launches_ms": round(t_single, 4),
             "per_problem_launches_tflops": round(flop / (t_single / 1e3) / 1e12, 2),
             "tile_policy": {"id": policy, "large_config": big, "small_config": small, "waste_bound": bound,

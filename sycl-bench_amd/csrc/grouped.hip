@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <numeric>
+#include <queue>
 
 #include "grouped.h"
 
@@ -352,6 +353,82 @@ std::vector<int> balance_segments(const std::vector<GProblem>& P, const std::vec
   return seg;
 }
 
+// Queue order of the one-kernel (in-kernel waits) mode when no problem has
+// pack items: a list schedule of every tile on `workers` workgroups.  The
+// next free workgroup takes a tile of the ready problem with the longest
+// remaining path to the end of the launch (its tile time plus its longest
+// consumer's, tile_ms estimates), ties to the larger K; a problem is ready
+// once its producers' tiles are scheduled to have ended.  So a chain's
+// later levels are queued as soon as their inputs are due, and the
+// independent problems fill the time around them, instead of every later
+// level waiting behind all first-level tiles.  A consumer's tiles follow all
+// of its producers' tiles in the list, so the order stays a topological
+// one: a workgroup that waits in-kernel only waits for tiles that other
+// workgroups have already taken.  Producers have lower indices than their
+// consumers (checked by validate).
+std::vector<GItem> critical_path_items(const std::vector<int>& at, const std::vector<GProblem>& P, int workers) {
+  const int count = (int)P.size();
+  std::vector<char> in(count, 0);
+  for(int i : at) in[i] = 1;
+  std::vector<std::vector<int>> cons(count);
+  std::vector<int> wait_for(count, 0);
+  for(int i : at)
+    for(int e = 0; e < P[i].ndeps; ++e)
+      if(in[P[i].deps[e]]) {
+        cons[P[i].deps[e]].push_back(i);
+        ++wait_for[i];
+      }
+  std::vector<double> dur(count, 0.0), path(count, 0.0);
+  for(int i : at) dur[i] = tile_ms(P[i].config, P[i].K);
+  for(int i = count - 1; i >= 0; --i) {
+    if(!in[i]) continue;
+    double longest = 0.0;
+    for(int c : cons[i]) longest = std::max(longest, path[c]);
+    path[i] = dur[i] + longest;
+  }
+  auto lower = [&](int a, int b) {  // a goes after b
+    if(path[a] != path[b]) return path[a] < path[b];
+    if(P[a].K != P[b].K) return P[a].K < P[b].K;
+    return a > b;
+  };
+  std::priority_queue<int, std::vector<int>, decltype(lower)> ready(lower);
+  using Due = std::pair<double, int>;  // (time its inputs are due, problem)
+  std::priority_queue<Due, std::vector<Due>, std::greater<Due>> pending;
+  std::vector<double> due(count, 0.0), finish(count, 0.0);
+  std::vector<int> next(count, 0);
+  for(int i : at)
+    if(wait_for[i] == 0) pending.push({0.0, i});
+  std::priority_queue<double, std::vector<double>, std::greater<double>> free_at;
+  for(int w = 0; w < std::max(1, workers); ++w) free_at.push(0.0);
+  std::vector<GItem> out;
+  while(!ready.empty() || !pending.empty()) {
+    double t = free_at.top();
+    free_at.pop();
+    while(!pending.empty() && pending.top().first <= t) {
+      ready.push(pending.top().second);
+      pending.pop();
+    }
+    if(ready.empty()) {  // nothing due yet: this workgroup waits for the earliest
+      t = pending.top().first;
+      ready.push(pending.top().second);
+      pending.pop();
+    }
+    const int p = ready.top();
+    const int tile = next[p]++;
+    out.push_back({ITEM_TILE, p, tile / P[p].tiles_n, tile % P[p].tiles_n});
+    finish[p] = std::max(finish[p], t + dur[p]);
+    free_at.push(t + dur[p]);
+    if(next[p] == P[p].tiles_m * P[p].tiles_n) {
+      ready.pop();
+      for(int c : cons[p]) {
+        due[c] = std::max(due[c], finish[p]);
+        if(--wait_for[c] == 0) pending.push({due[c], c});
+      }
+    }
+  }
+  return out;
+}
+
 // Tiles of one problem that go to one queue: at most one round of an XCD's
 // CUs (kQueueGroupTiles), whole tile rows where possible, so the group's
 // tiles share their A panels (and B panels) in that XCD's L2.
@@ -586,8 +663,16 @@ mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* p
         }
     prepack_seg_end.push_back((int)prepack_ops.size());
     const size_t first_item = pl.items.size();
-    if(!pl.per_level) {
-      // one queue, dependency level by level
+    bool no_pack_items = true;
+    for(int i : at) no_pack_items = no_pack_items && (!packed || pl.problems[i].prepacked);
+    if(!pl.per_level && no_pack_items) {
+      // one queue in list-schedule order
+      const auto order = critical_path_items(at, pl.problems, cus);
+      pl.items.insert(pl.items.end(), order.begin(), order.end());
+      pl.qbeg.push_back(0);
+      for(int q = 0; q < GROUPED_QUEUES; ++q) pl.qbeg.push_back((int)(pl.items.size() - first_item));
+    } else if(!pl.per_level) {
+      // one queue, dependency level by level (pack items before their level's tiles)
       for(int lv = 0; lv <= max_level; ++lv) {
         if(packed)
           for(int i : at) {
