@@ -698,7 +698,9 @@ def main():
         out["configs"] = measure_configs(torch, mfx, h)
     dims = None
     if not args.no_grouped:
-        g, dims = grouped_workload(torch, mfx, h, max(3, args.steps), max(1, args.warmup))
+        # replays of ~1.6 ms: at least 20 timed after 10 untimed (the autotuner's
+        # sweep right before leaves the clock low for the first few)
+        g, dims = grouped_workload(torch, mfx, h, max(20, args.steps), max(10, args.warmup))
         g["frac_of_large_tile"] = round(g["tflops"] / value, 3)
         out["grouped"] = g
         log(f"grouped: {g}")
