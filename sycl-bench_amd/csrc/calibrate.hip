@@ -1,5 +1,5 @@
 // Roofline calibrators: the measured ceilings the contraction kernels are
-// judged against.  The methodology follows micro/arith.cpp:41-68 (a
+// compared with.  The methodology follows micro/arith.cpp:41-68 (a
 // register-only loop of dependent arithmetic whose result is known exactly)
 // and micro/DRAM.cpp:47-62 (a device copy, read + write bytes per second),
 // on the units that bound this path: the matrix cores and HBM.
