@@ -280,12 +280,18 @@ def measure_configs(torch, mfx, h, reps=3):
     A, B, C0 = seeded((n, n), 0), seeded((n, n), 1), seeded((n, n), 2)
     R = C0.double()
     h.gemm_f64(A.double(), B.double(), R, alpha=1.5, beta=1.2)
+    # A 25-us kernel: the clock ramps up over the first ~1000 back-to-back
+    # calls (~25 ms; measured 27 us per call after 100, 24.4 us from 1000 on,
+    # also straight after N=8192 work), so 1000 untimed calls go to a scratch
+    # C before 200 timed ones start from a fresh copy (C grows by 1.2x per
+    # call: 200 calls stay far inside FP32 range).
+    Cw = C0.clone()
+    for _ in range(1000):
+        h.gemm(A, B, Cw, alpha=1.5, beta=1.2, prec="fp32")
+    del Cw
     C = C0.clone()
     g1024 = lambda: h.gemm(A, B, C, alpha=1.5, beta=1.2, prec="fp32")
-    # A 25-us kernel right after the N=8192 runs: 100 untimed calls let the
-    # clock settle from the heavy workloads before 200 timed ones (C grows by
-    # 1.2x per call; 300 calls stay far inside FP32 range).
-    ms = _time_ms(torch, g1024, 200, warm=100)
+    ms = _time_ms(torch, g1024, 200, warm=0)
     C = C0.clone()
     h.gemm(A, B, C, alpha=1.5, beta=1.2, prec="fp32")
     rec("gemm_1024_fp32", 2.0 * n ** 3, ms, h.rel_frobenius(C, R), g1024)
