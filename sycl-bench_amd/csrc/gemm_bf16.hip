@@ -59,7 +59,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16x_kernel(
     grouped_tile(pid, p.tiles_m, p.tiles_n, p.group, tm, tn);
   } else {
     int ti, tj;
-    lower_tile(pid, ti, tj);
+    grouped_lower_tile(pid, p.tiles_m, p.group, ti, tj);
     if(p.mode == TILES_LOWER) { tm = ti; tn = tj; } else { tm = tj; tn = ti; }
   }
   bf16_tile<BM, BN, WM, WN, NHL>(p.t, tm, tn, lds);

@@ -26,7 +26,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * KG * 64) void gemm_f32_kern
     grouped_tile(pid, p.tiles_m, p.tiles_n, p.group, tm, tn);
   } else {
     int ti, tj;
-    lower_tile(pid, ti, tj);
+    grouped_lower_tile(pid, p.tiles_m, p.group, ti, tj);
     if(p.op.tiles == TILES_LOWER) { tm = ti; tn = tj; } else { tm = tj; tn = ti; }
   }
   f32_tile<BM, BN, BK, WM, WN, AK, BKM, VEC, KG>(p.op, tm, tn, smem);

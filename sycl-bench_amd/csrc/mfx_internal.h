@@ -221,13 +221,36 @@ __device__ __forceinline__ void grouped_tile(int pid, int tiles_m, int tiles_n, 
   tn = r / gsize;
 }
 
-// Tile (ti, tj) with tj <= ti for linear index p over the lower triangle.
-__device__ __forceinline__ void lower_tile(int p, int& ti, int& tj) {
-  int i = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
-  while(i * (i + 1) / 2 > p) --i;
-  while((i + 1) * (i + 2) / 2 <= p) ++i;
-  ti = i;
-  tj = p - i * (i + 1) / 2;
+// The lower triangle of a T x T tile grid in bands of `group` tile rows, each
+// band walked column by column (its full-width part, then its own triangle),
+// so consecutive ids -- one XCD's workgroups after xcd_remap -- cover a
+// compact block of tiles that shares A and B panels in L2, as grouped_tile
+// does for a full grid.  A plain row-by-row order gives each round of an
+// XCD's workgroups one or two tile rows and as many distinct B panels as
+// tiles.  Every (ti, tj) with tj <= ti < T once for
+// p < T (T + 1) / 2.
+__host__ __device__ inline void grouped_lower_tile(int p, int T, int group, int& ti, int& tj) {
+  int r0 = 0, rows = group < T ? group : T;
+  for(;;) {
+    const int in_band = rows * r0 + rows * (rows + 1) / 2;
+    if(p < in_band || r0 + rows >= T) break;
+    p -= in_band;
+    r0 += rows;
+    rows = group < T - r0 ? group : T - r0;
+  }
+  if(p < rows * r0) {
+    tj = p / rows;
+    ti = r0 + p % rows;
+    return;
+  }
+  p -= rows * r0;
+  int c = 0;
+  while(p >= rows - c) {
+    p -= rows - c;
+    ++c;
+  }
+  tj = r0 + c;
+  ti = r0 + c + p;
 }
 
 // Scheduling modes shared by the FP32 and BF16 kernel families.

@@ -234,7 +234,25 @@ static void check_plan(const Case& cs, mfx_precision prec, int policy) {
   }
 }
 
+// The triangular schedules' tile order: a bijection onto the lower triangle.
+static void check_lower_order() {
+  const std::string ctx = "triangle order";
+  for(int group : {1, 3, 8})
+    for(int T = 1; T <= 70; ++T) {
+      const int n = T * (T + 1) / 2;
+      std::set<std::pair<int, int>> seen;
+      for(int p = 0; p < n; ++p) {
+        int ti = -1, tj = -1;
+        grouped_lower_tile(p, T, group, ti, tj);
+        CHECK(tj >= 0 && tj <= ti && ti < T, "T %d group %d: p %d -> (%d, %d) outside the triangle", T, group, p, ti, tj);
+        CHECK(seen.insert({ti, tj}).second, "T %d group %d: (%d, %d) twice", T, group, ti, tj);
+      }
+      CHECK((int)seen.size() == n, "T %d group %d: %zu of %d tiles", T, group, seen.size(), n);
+    }
+}
+
 int main() {
+  check_lower_order();
   std::vector<Case> cases;
   {
     // the batched workload: uneven problems (log-uniform dims, multiples of
