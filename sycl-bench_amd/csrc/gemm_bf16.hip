@@ -48,6 +48,7 @@ struct BfParams {
   BfTileArgs t;
   int tiles_m, tiles_n, group;
   int mode;  // TILES_FULL / LOWER / UPPER
+  int tail_first = -1, tail_ratio = 1, big_tiles = 0;  // triangle_tile's re-tiled last round
 };
 
 template <int BM, int BN, int WM, int WN, int NHL>
@@ -59,7 +60,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16x_kernel(
     grouped_tile(pid, p.tiles_m, p.tiles_n, p.group, tm, tn);
   } else {
     int ti, tj;
-    grouped_lower_tile(pid, p.tiles_m, p.group, ti, tj);
+    if(!triangle_tile(pid, p.tiles_m, p.group, p.tail_first, p.tail_ratio, p.big_tiles, ti, tj)) return;
     if(p.mode == TILES_LOWER) { tm = ti; tn = tj; } else { tm = tj; tn = ti; }
   }
   bf16_tile<BM, BN, WM, WN, NHL>(p.t, tm, tn, lds);
@@ -88,6 +89,19 @@ mfx_status launch_bf_cfg(Handle* h, const BfParams& p, bool split, int grid, hip
   return timed_launch(h, "gemm_bf16_mfma", s, [&] {
     gemm_bf16x_kernel<BM, BN, WM, WN, 1><<<grid, BfCfg<BM, BN, WM, WN, 1>::NT, 0, s>>>(p);
   });
+}
+
+mfx_status launch_bf_config(Handle* h, const BfParams& p, bool split, int config, int grid, hipStream_t s) {
+  switch(config) {
+    case 0: return launch_bf_cfg<256, 256, 128, 128>(h, p, split, grid, s);
+    case 1: return launch_bf_cfg<128, 128, 64, 64>(h, p, split, grid, s);
+    case 2: return launch_bf_cfg<64, 64, 32, 32>(h, p, split, grid, s);
+    case 3: return launch_bf_cfg<256, 128, 128, 64>(h, p, split, grid, s);
+    case 4: return launch_bf_cfg<256, 256, 128, 64>(h, p, split, grid, s);
+    case 5: return launch_bf_cfg<256, 128, 64, 64>(h, p, split, grid, s);
+    case 6: return launch_bf_cfg<128, 128, 64, 32>(h, p, split, grid, s);
+  }
+  return invalid("unknown BF16 configuration");
 }
 
 struct PackPlan {
@@ -203,16 +217,22 @@ mfx_status launch_dense_bf16x_ws(Handle* h, const DenseOp& op, bool split, int c
     if(p.tiles_m != p.tiles_n || c.bm != c.bn) return invalid("triangular schedule needs square tiles");
     grid = p.tiles_m * (p.tiles_m + 1) / 2;
   }
-  switch(config) {
-    case 0: return launch_bf_cfg<256, 256, 128, 128>(h, p, split, grid, s);
-    case 1: return launch_bf_cfg<128, 128, 64, 64>(h, p, split, grid, s);
-    case 2: return launch_bf_cfg<64, 64, 32, 32>(h, p, split, grid, s);
-    case 3: return launch_bf_cfg<256, 128, 128, 64>(h, p, split, grid, s);
-    case 4: return launch_bf_cfg<256, 256, 128, 64>(h, p, split, grid, s);
-    case 5: return launch_bf_cfg<256, 128, 64, 64>(h, p, split, grid, s);
-    case 6: return launch_bf_cfg<128, 128, 64, 32>(h, p, split, grid, s);
+  if(grid == 0) return MFX_SUCCESS;
+  if(op.tiles != TILES_FULL) {
+    const TriTail tail =
+        tri_tail(split ? MFX_PREC_SPLIT_BF16 : MFX_PREC_BF16, config, p.tiles_m, h->cu_count > 0 ? h->cu_count : 256);
+    if(tail.small >= 0) {
+      // the full rounds on this tile, the last round's big tiles re-tiled
+      MFX_CHECK(launch_bf_config(h, p, split, config, tail.head, s));
+      BfParams q = p;
+      q.tiles_m = q.tiles_n = (int)ceil_div(op.M, kBfConfigs[tail.small].bm);
+      q.tail_first = tail.head;
+      q.tail_ratio = tail.ratio;
+      q.big_tiles = p.tiles_m;
+      return launch_bf_config(h, q, split, tail.small, (grid - tail.head) * tail.ratio * tail.ratio, s);
+    }
   }
-  return invalid("unknown BF16 configuration");
+  return launch_bf_config(h, p, split, config, grid, s);
 }
 
 // Element-wise split used by mfx_split_bf16 (a test and tooling utility).

@@ -15,6 +15,7 @@ struct F32Params {
   int tiles_m, tiles_n;
   int group;
   int vec_a, vec_b;  // 128-bit global loads allowed for A / B
+  int tail_first = -1, tail_ratio = 1, big_tiles = 0;  // triangle_tile's re-tiled last round
 };
 
 template <int BM, int BN, int BK, int WM, int WN, int KG, bool AK, bool BKM, bool VEC>
@@ -26,7 +27,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * KG * 64) void gemm_f32_kern
     grouped_tile(pid, p.tiles_m, p.tiles_n, p.group, tm, tn);
   } else {
     int ti, tj;
-    grouped_lower_tile(pid, p.tiles_m, p.group, ti, tj);
+    if(!triangle_tile(pid, p.tiles_m, p.group, p.tail_first, p.tail_ratio, p.big_tiles, ti, tj)) return;
     if(p.op.tiles == TILES_LOWER) { tm = ti; tn = tj; } else { tm = tj; tn = ti; }
   }
   f32_tile<BM, BN, BK, WM, WN, AK, BKM, VEC, KG>(p.op, tm, tn, smem);
@@ -222,6 +223,20 @@ bool f32_config_info(int c, int* bm, int* bn, int* bk, int* waves) {
   return true;
 }
 
+static mfx_status launch_f32_config(Handle* h, const F32Params& p, int config, int grid, hipStream_t s) {
+  switch(config) {
+    case 0: return launch_cfg<256, 256, 16, 128, 128>(h, p, grid, s);
+    case 1: return launch_cfg<128, 128, 16, 64, 64>(h, p, grid, s);
+    case 2: return launch_cfg<64, 64, 32, 32, 32>(h, p, grid, s);
+    case 3: return launch_cfg<256, 128, 16, 128, 64>(h, p, grid, s);
+    case 4: return launch_cfg<256, 256, 16, 128, 64>(h, p, grid, s);
+    case 5: return launch_cfg<256, 128, 16, 64, 64>(h, p, grid, s);
+    case 6: return launch_cfg<128, 128, 16, 64, 32>(h, p, grid, s);
+    case 7: return launch_cfg<64, 64, 64, 32, 32, 4>(h, p, grid, s);
+  }
+  return invalid("unknown FP32 configuration");
+}
+
 // k slices for this call: the handle's override, else the automatic choice;
 // 1 for triangular schedules and two-product (syr2k) contractions.
 static int f32_call_splits(const Handle* h, const DenseOp& op, int config, int64_t tiles) {
@@ -255,13 +270,26 @@ mfx_status launch_dense_f32_ws(Handle* h, const DenseOp& op, int config, void* w
     p.vec_a = p.vec_b = v;
   }
   int grid;
+  TriTail tail;
   if(op.tiles == TILES_FULL) {
     grid = p.tiles_m * p.tiles_n;
   } else {
     if(p.tiles_m != p.tiles_n || c.bm != c.bn) return invalid("triangular schedule needs square tiles");
     grid = p.tiles_m * (p.tiles_m + 1) / 2;
+    tail = tri_tail(MFX_PREC_FP32, config, p.tiles_m, h->cu_count > 0 ? h->cu_count : 256);
   }
   if(grid == 0) return MFX_SUCCESS;
+  if(tail.small >= 0) {
+    // the full rounds on this tile, the last round's big tiles re-tiled
+    MFX_CHECK(launch_f32_config(h, p, config, tail.head, s));
+    F32Params q = p;
+    const int sb = kF32Configs[tail.small].bm;
+    q.tiles_m = q.tiles_n = (int)ceil_div(op.M, sb);
+    q.tail_first = tail.head;
+    q.tail_ratio = tail.ratio;
+    q.big_tiles = p.tiles_m;
+    return launch_f32_config(h, q, tail.small, (grid - tail.head) * tail.ratio * tail.ratio, s);
+  }
   const int splits = f32_call_splits(h, op, config, grid);
   if(splits > 1) {
     const size_t need = f32_splitk_workspace(config, grid, splits);
@@ -272,17 +300,7 @@ mfx_status launch_dense_f32_ws(Handle* h, const DenseOp& op, int config, void* w
     }
     return launch_f32_splitk(h, op, config, splits, static_cast<float*>(ws), s);
   }
-  switch(config) {
-    case 0: return launch_cfg<256, 256, 16, 128, 128>(h, p, grid, s);
-    case 1: return launch_cfg<128, 128, 16, 64, 64>(h, p, grid, s);
-    case 2: return launch_cfg<64, 64, 32, 32, 32>(h, p, grid, s);
-    case 3: return launch_cfg<256, 128, 16, 128, 64>(h, p, grid, s);
-    case 4: return launch_cfg<256, 256, 16, 128, 64>(h, p, grid, s);
-    case 5: return launch_cfg<256, 128, 16, 64, 64>(h, p, grid, s);
-    case 6: return launch_cfg<128, 128, 16, 64, 32>(h, p, grid, s);
-    case 7: return launch_cfg<64, 64, 64, 32, 32, 4>(h, p, grid, s);
-  }
-  return invalid("unknown FP32 configuration");
+  return launch_f32_config(h, p, config, grid, s);
 }
 
 }  // namespace mfx

@@ -240,9 +240,99 @@ int tuned_lookup(const std::string& key) {
   return it == g_tuned.end() ? -1 : it->second;
 }
 
+// The square configuration a triangle's sparse last round is re-tiled with:
+// half or a quarter of the big tile's side.
+static int tri_small_config(mfx_precision p, int side) {
+  if(p == MFX_PREC_FP32) return side == 128 ? 6 : side == 64 ? 7 : -1;
+  if(p == MFX_PREC_SPLIT_BF16 || p == MFX_PREC_BF16) return side == 128 ? 6 : side == 64 ? 2 : -1;
+  return -1;
+}
+
+TriTail tri_tail(mfx_precision p, int config, int64_t T, int cus) {
+  TriTail t;
+  int bm, bn, bk, w;
+  if(!config_info(p, config, &bm, &bn, &bk, &w) || bm != bn || cus <= 0) return t;
+  const int64_t tiles = T * (T + 1) / 2, rounds = tiles / cus, tail = tiles - rounds * cus;
+  t.head = (int)tiles;
+  if(rounds == 0 || tail == 0 || 2 * tail > cus) return t;
+  // one more round of big tiles, against ceil(tail r^2 / cus) rounds of small ones
+  double best = (double)bm * bm / config_rate(p, config);
+  for(int r : {2, 4}) {
+    const int s = tri_small_config(p, bm / r);
+    if(s < 0) continue;
+    const double cost = (double)ceil_div(tail * r * r, cus) * (double)(bm / r) * (bm / r) / config_rate(p, s);
+    if(cost < best) {
+      best = cost;
+      t.small = s;
+      t.ratio = r;
+    }
+  }
+  if(t.small >= 0) t.head = (int)(rounds * cus);
+  return t;
+}
+
+// Least estimated time over the square configurations for an n x n
+// triangle: rounds of T(T+1)/2 tiles over the CUs, each costing the tile's
+// area over its rate, with a sparse last round re-tiled as tri_tail does.
+// BF16 / FP8 take their largest tile when it fills between half and all of
+// the CUs in one round (the power-limited clock rises as CUs idle; see
+// least_time_config).
+static int least_time_triangle(Handle* h, mfx_precision p, int64_t n) {
+  const int cus = h->cu_count > 0 ? h->cu_count : 256;
+  auto tri = [&](int b) {
+    const int64_t t = ceil_div(n, b);
+    return t * (t + 1) / 2;
+  };
+  int best = -1, best_w = 0;
+  double best_t = 0.0;
+  int64_t best_area = 0;
+  if(p != MFX_PREC_FP32) {
+    for(int c = 0; c < config_count(p); ++c) {
+      int bm, bn, bk, w;
+      if(!config_info(p, c, &bm, &bn, &bk, &w) || bm != bn) continue;
+      if((int64_t)bm * bn > best_area || ((int64_t)bm * bn == best_area && w > best_w)) {
+        best = c;
+        best_area = (int64_t)bm * bn;
+        best_w = w;
+      }
+    }
+    if(best >= 0) {
+      int bm, bn, bk, w;
+      config_info(p, best, &bm, &bn, &bk, &w);
+      const int64_t t = tri(bm);
+      if(2 * t >= cus && t <= cus) return best;
+    }
+    best = -1;
+    best_area = 0;
+    best_w = 0;
+  }
+  for(int c = 0; c < config_count(p); ++c) {
+    int bm, bn, bk, w;
+    if(!config_info(p, c, &bm, &bn, &bk, &w) || bm != bn) continue;
+    const int64_t T = ceil_div(n, bm), tiles = tri(bm), area = (int64_t)bm * bn;
+    const TriTail tt = tri_tail(p, c, T, cus);
+    double t;
+    if(tt.small >= 0) {
+      const int sb = bm / tt.ratio;
+      t = (double)(tiles / cus) * area / config_rate(p, c) +
+          (double)ceil_div((tiles - tt.head) * tt.ratio * tt.ratio, cus) * sb * sb / config_rate(p, tt.small);
+    } else {
+      t = (double)ceil_div(tiles, cus) * area / config_rate(p, c);
+    }
+    const bool better = best < 0 || t < best_t * (1.0 - 1e-9) ||
+                        (t <= best_t * (1.0 + 1e-9) && (area > best_area || (area == best_area && w > best_w)));
+    if(better) {
+      best = c;
+      best_t = t;
+      best_area = area;
+      best_w = w;
+    }
+  }
+  return best < 0 ? 0 : best;
+}
+
 // Square-tile choice for the triangular schedules: a forced or tuned square
-// configuration, else the least estimated time over the T(T+1)/2 triangle
-// tiles.
+// configuration, else least_time_triangle.
 int choose_square_config(Handle* h, mfx_precision p, int64_t n, int64_t k) {
   int c = -1;
   if(h->forced_config[p] >= 0) {
@@ -254,10 +344,7 @@ int choose_square_config(Handle* h, mfx_precision p, int64_t n, int64_t k) {
   }
   int bm, bn, bk, w;
   if(c >= 0) return config_info(p, c, &bm, &bn, &bk, &w) && bm == bn ? c : 0;
-  return least_time_config(h, p, true, [&](int b, int) {
-    const int64_t t = ceil_div(n, b);
-    return t * (t + 1) / 2;
-  });
+  return least_time_triangle(h, p, n);
 }
 
 mfx_status dispatch_dense(Handle* h, mfx_precision prec, DenseOp& op, int config, hipStream_t s) {

@@ -253,6 +253,26 @@ __host__ __device__ inline void grouped_lower_tile(int p, int T, int group, int&
   ti = r0 + c + p;
 }
 
+// A triangular launch's tile (ti, tj) for id p: the triangle order of the
+// T x T grid, or -- when `tail_first` >= 0, the re-tiled last round of a
+// bigger tile's triangle (tri_tail) -- sub-tile p % r^2 of big tile
+// tail_first + p / r^2 of the big_T x big_T triangle.  False for a sub-tile
+// with nothing of the triangle in it (above a diagonal big tile's diagonal,
+// or past the matrix), which the workgroup skips.
+__host__ __device__ inline bool triangle_tile(int p, int T, int group, int tail_first, int r, int big_T, int& ti,
+    int& tj) {
+  if(tail_first < 0) {
+    grouped_lower_tile(p, T, group, ti, tj);
+    return true;
+  }
+  int bi, bj;
+  grouped_lower_tile(tail_first + p / (r * r), big_T, group, bi, bj);
+  const int s = p % (r * r);
+  ti = bi * r + s / r;
+  tj = bj * r + s % r;
+  return tj <= ti && ti < T;
+}
+
 // Scheduling modes shared by the FP32 and BF16 kernel families.
 enum TileMode : int { TILES_FULL = 0, TILES_LOWER = 1, TILES_UPPER = 2 };
 
@@ -353,6 +373,18 @@ struct DenseOp {
   Epilogue ep;
   int tiles;                                    // TILES_FULL / LOWER / UPPER
 };
+
+// Triangular schedules whose last round of tiles leaves most CUs idle: the
+// full rounds run on the chosen square tile, and the big tiles of the last
+// round are re-tiled `ratio` x `ratio` into a smaller square configuration
+// in a second launch, when that is estimated faster than one more round of
+// big tiles (small == -1: no split).
+struct TriTail {
+  int head = 0;    // big tiles of the first launch (all of them without a split)
+  int small = -1;  // configuration of the second launch
+  int ratio = 1;   // big tile side / small tile side
+};
+TriTail tri_tail(mfx_precision p, int config, int64_t tiles_per_side, int cus);
 
 mfx_status launch_dense_f32(Handle* h, const DenseOp& op, int config, hipStream_t s);
 // The same with split-K partial tiles in `ws` (ws_bytes long) instead of the

@@ -251,8 +251,61 @@ static void check_lower_order() {
     }
 }
 
+// A triangle whose sparse last round is re-tiled (tri_tail): every small
+// tile of the lower triangle is computed exactly once, by a big tile of the
+// first launch or by a sub-tile of the second.
+static void check_triangle_tail() {
+  const std::string ctx = "triangle tail";
+  int splits = 0;
+  for(mfx_precision p : {MFX_PREC_FP32, MFX_PREC_SPLIT_BF16, MFX_PREC_BF16})
+    for(int cus : {256, 304, 80})
+      for(int64_t n : {1000, 2048, 4096, 5000, 8192, 8200, 12288}) {
+        for(int c = 0; c < config_count(p); ++c) {
+          int bm, bn, bk, w;
+          if(!config_info(p, c, &bm, &bn, &bk, &w) || bm != bn) continue;
+          const int T = (int)ceil_div(n, bm);
+          const TriTail tt = tri_tail(p, c, T, cus);
+          const int tiles = T * (T + 1) / 2;
+          if(tt.small < 0) {
+            CHECK(tt.head == tiles, "no split but head %d of %d tiles", tt.head, tiles);
+            continue;
+          }
+          ++splits;
+          int sm, sn, sk, sw;
+          CHECK(config_info(p, tt.small, &sm, &sn, &sk, &sw) && sm == sn && sm * tt.ratio == bm,
+              "small configuration %d for %d / %d", tt.small, bm, tt.ratio);
+          CHECK(tt.head % cus == 0 && tt.head > 0 && tt.head < tiles, "head %d of %d tiles", tt.head, tiles);
+          const int r = tt.ratio, Ts = (int)ceil_div(n, bm / r);
+          std::map<std::pair<int, int>, int> cover;
+          for(int q = 0; q < tt.head; ++q) {
+            int bi, bj;
+            grouped_lower_tile(q, T, 8, bi, bj);
+            for(int a = 0; a < r; ++a)
+              for(int b = 0; b < r; ++b)
+                if(bj * r + b <= bi * r + a && bi * r + a < Ts) ++cover[{bi * r + a, bj * r + b}];
+          }
+          for(int q = 0; q < (tiles - tt.head) * r * r; ++q) {
+            int ti, tj;
+            if(triangle_tile(q, Ts, 8, tt.head, r, T, ti, tj)) ++cover[{ti, tj}];
+          }
+          int missing = 0, twice = 0;
+          for(int i = 0; i < Ts; ++i)
+            for(int j = 0; j <= i; ++j) {
+              auto it = cover.find({i, j});
+              if(it == cover.end()) ++missing;
+              else if(it->second != 1) ++twice;
+            }
+          CHECK(missing == 0 && twice == 0 && (int)cover.size() == Ts * (Ts + 1) / 2,
+              "prec %d cus %d n %lld config %d: %d small tiles missing, %d covered twice", (int)p, cus, (long long)n, c,
+              missing, twice);
+        }
+      }
+  CHECK(splits > 10, "only %d triangles split", splits);
+}
+
 int main() {
   check_lower_order();
+  check_triangle_tail();
   std::vector<Case> cases;
   {
     // the batched workload: uneven problems (log-uniform dims, multiples of
