@@ -243,16 +243,18 @@ def _time_ms(torch, fn, reps, warm=1):
 
 
 def _kernel_ms(torch, h, fn, reps):
-    """Average duration (ms) of each kernel `fn` launches, from the library's
-    hipEvent pair around every launch, over `reps` calls (a separate pass from
-    the end-to-end timing, which the event records would perturb)."""
+    """Per kernel name that `fn` launches: [ms per call, launches per call],
+    from the library's hipEvent pair around every launch, over `reps` calls
+    (a separate pass from the end-to-end timing, which the event records
+    would perturb).  A call can launch a kernel more than once (2mm: two
+    products; a triangle whose last round is re-tiled: two launches)."""
     fn()
     torch.cuda.synchronize()
     h.timing_begin()
     for _ in range(reps):
         fn()
     torch.cuda.synchronize()
-    return {k: round(v[1] / v[0], 4) for k, v in h.timing_end().items()}
+    return {k: [round(v[1] / reps, 4), round(v[0] / reps, 2)] for k, v in h.timing_end().items()}
 
 
 def measure_configs(torch, mfx, h, reps=3):
@@ -272,7 +274,7 @@ def measure_configs(torch, mfx, h, reps=3):
     def rec(name, work, ms, err, fn=None):
         out[name] = {"ms": round(ms, 4), "tflops": round(work / (ms / 1e3) / 1e12, 1), "rel_err": err}
         if fn is not None:
-            out[name]["kernel_avg_ms"] = _kernel_ms(torch, h, fn, reps)
+            out[name]["kernel_ms_per_call"] = _kernel_ms(torch, h, fn, reps)
         log(f"configs: {name}: {out[name]}")
 
     # gemm N=1024, FP32 (alpha 1.5, beta 1.2)
