@@ -319,8 +319,10 @@ mfx_status mfx_autotune(mfx_handle handle, mfx_precision prec, int64_t m, int64_
  * times the launch under every policy (see mfx_grouped_policy_info), records
  * the fastest for launches with the same key (precision, problem count,
  * contraction volume and dependency depth, in power-of-two buckets) and
- * returns its id.  Runs the launch several times on the given buffers (C is
- * overwritten).  Blocking.
+ * returns its id.  Each policy's launch is planned once and timed over
+ * back-to-back launches of the same tables (as a captured graph replays it),
+ * in two rounds over the policies, the faster kept.  Runs the launch several
+ * times on the given buffers (C is overwritten).  Blocking.
  */
 mfx_status mfx_autotune_grouped(mfx_handle handle, mfx_precision prec,
     const mfx_gemm_problem* problems, int32_t count, const mfx_dep* deps, int32_t ndeps,

@@ -259,6 +259,13 @@ struct GroupedPlan {
 mfx_status plan_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* problems, int count,
     const mfx_dep* deps, int ndeps, GroupedPlan& pl);
 
+// Plans and binds a grouped launch once, launches it once untimed, then
+// returns in *ms the average time of `reps` back-to-back launches of the same
+// bound tables (what a captured graph replays: no host planning between
+// launches, so the GPU never idles between them).
+mfx_status time_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* problems, int count,
+    const mfx_dep* deps, int ndeps, int reps, hipStream_t s, double* ms);
+
 // Pre-pass: packs every job's operand (one chunk per thread).
 mfx_status launch_pack_jobs(Handle* h, int nhl, const PackJob* jobs, int njobs, int blocks, hipStream_t s);
 

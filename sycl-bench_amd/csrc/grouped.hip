@@ -888,6 +888,36 @@ mfx_status run_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* pr
   return launch_grouped_kernels(h, pl, g, s);
 }
 
+mfx_status time_grouped(Handle* h, mfx_precision prec, const mfx_gemm_problem* problems, int count,
+    const mfx_dep* deps, int ndeps, int reps, hipStream_t s, double* ms) {
+  GroupedPlan pl;
+  MFX_CHECK(plan_grouped(h, prec, problems, count, deps, ndeps, pl));
+  void* ws = nullptr;
+  MFX_CHECK(workspace(h, pl.total_bytes, &ws));
+  GroupedArgs g;
+  MFX_CHECK(bind_grouped(h, pl, ws, g, s));
+  if(!h->error_flag) MFX_HIP(hipMalloc(&h->error_flag, 256));
+  g.error = h->error_flag;
+  MFX_CHECK(launch_grouped_kernels(h, pl, g, s));  // untimed
+  hipEvent_t a = nullptr, b = nullptr;
+  MFX_HIP(hipEventCreate(&a));
+  hipError_t e = hipEventCreate(&b);
+  mfx_status st = e == hipSuccess ? MFX_SUCCESS : hip_fail(e, "hipEventCreate");
+  if(st == MFX_SUCCESS && (e = hipEventRecord(a, s)) != hipSuccess) st = hip_fail(e, "hipEventRecord");
+  for(int r = 0; r < reps && st == MFX_SUCCESS; ++r) st = launch_grouped_kernels(h, pl, g, s);
+  float t = 0.0f;
+  if(st == MFX_SUCCESS) {
+    e = hipEventRecord(b, s);
+    if(e == hipSuccess) e = hipEventSynchronize(b);
+    if(e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+    if(e != hipSuccess) st = hip_fail(e, "grouped timing");
+  }
+  (void)hipEventDestroy(a);
+  if(b) (void)hipEventDestroy(b);
+  *ms = t / std::max(reps, 1);
+  return st;
+}
+
 // Balanced-tree product of `count` n x n matrices as grouped problems.
 mfx_status chain_problems(int count, const float* const* mats, int64_t n, float* const* scratch, float* out,
     std::vector<mfx_gemm_problem>& probs, std::vector<mfx_dep>& deps) {

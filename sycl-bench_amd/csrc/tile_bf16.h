@@ -590,8 +590,19 @@ __device__ __forceinline__ void bf16_tile_direct(const DirectTileArgs& t, int tm
   constexpr int NT = Cfg::NT, NW = Cfg::NW, FM = Cfg::FM, FN = Cfg::FN;
   constexpr int SA = Cfg::CH_A, SB = Cfg::CH_B;
   constexpr int PER = Cfg::PER_A + Cfg::PER_B;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int wm = wid / Cfg::NWN, wn = wid % Cfg::NWN;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // The WM x WN sub-tiles that meet the problem go to the first waves, row by
+  // row (a full tile: wave w takes sub-tile w, as everywhere else); the waves
+  // past them still issue their share of every stage's loads but skip the
+  // split, the MFMAs and the store.  A tile that covers a fraction of its
+  // area (small and ragged problems) then keeps fewer waves per SIMD busy
+  // instead of contracting padding.  Which wave computes a sub-tile does not
+  // change its MFMA sequence, so results are bit-identical.
+  const int act_m = min(Cfg::NWM, (t.ep.M - tm * BM + WM - 1) / WM);
+  const int act_n = min(Cfg::NWN, (t.ep.N - tn * BN + WN - 1) / WN);
+  const bool active = wid < act_m * act_n;
+  const int wm = active ? wid / act_n : 0, wn = active ? wid % act_n : 0;
 
   floatx16 acc[FM][FN];
   zero_acc(acc);
@@ -660,10 +671,10 @@ __device__ __forceinline__ void bf16_tile_direct(const DirectTileArgs& t, int tm
     __syncthreads();
     if(s + NB - 1 < steps) issue(s + NB - 1);
     const u32x4* sa = lds + (s % NB) * (SA + SB);
-    contract(sa, sa + SA);
+    if(active) contract(sa, sa + SA);
   }
 
-  static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
+  if(active) static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
     constexpr int i = decltype(ij)::value / FN, j = decltype(ij)::value % FN;
     store_block32(t.ep, acc[i][j], tm * BM + wm * WM + i * 32, tn * BN + wn * WN + j * 32, lane);
   });
