@@ -567,9 +567,12 @@ __device__ __forceinline__ const float* direct_src(const DirectOp& o, int row0, 
   return o.p + (int64_t)(h * 8 + (l >> 3)) * o.ld + c;
 }
 
-// Lane l's fragment of block blk (8 consecutive k of its row) from a stage.
-__device__ __forceinline__ void direct_frag(const u32x4* st, int kmajor, int blk, int l, float (&x)[8]) {
-  if(kmajor) {
+// Lane l's fragment of block blk (8 consecutive k of its row) from a stage;
+// KMAJOR is the operand's layout (a compile-time choice, so a stage's reads,
+// splits and MFMAs form one block the scheduler can interleave).
+template <bool KMAJOR>
+__device__ __forceinline__ void direct_frag(const u32x4* st, int blk, int l, float (&x)[8]) {
+  if constexpr(KMAJOR) {
     const floatx4 u = __builtin_bit_cast(floatx4, st[blk * 128 + l]);
     const floatx4 v = __builtin_bit_cast(floatx4, st[blk * 128 + 64 + l]);
 #pragma unroll
@@ -581,6 +584,20 @@ __device__ __forceinline__ void direct_frag(const u32x4* st, int kmajor, int blk
     const float* f = reinterpret_cast<const float*>(st + blk * 128);
 #pragma unroll
     for(int e = 0; e < 8; ++e) x[e] = f[(8 * (l >> 5) + e) * 32 + (l & 31)];
+  }
+}
+
+// Pair p (elements 2p, 2p+1) of split_chunk: the same operations, so a
+// fragment split pair by pair equals split_chunk's.
+template <int NHL>
+__device__ __forceinline__ void split_pair(const float (&v)[8], int p, uint32_t& h, uint32_t& l) {
+  h = cvt_pk_bf16(v[2 * p], v[2 * p + 1]);
+  if(NHL == 2) {
+    const float h0 = __builtin_bit_cast(float, h << 16);
+    const float h1 = __builtin_bit_cast(float, h & 0xffff0000u);
+    l = cvt_pk_bf16(v[2 * p] - h0, v[2 * p + 1] - h1);
+  } else {
+    l = 0;
   }
 }
 
@@ -627,51 +644,83 @@ __device__ __forceinline__ void bf16_tile_direct(const DirectTileArgs& t, int tm
     for(int i = 0; i < Cfg::PER_B; ++i)
       lds_dma16(reinterpret_cast<const u32x4*>(src_b[i] + s * step_b), sb + wave_base + i * NT);
   };
-  auto contract = [&](const u32x4* sa, const u32x4* sb) __attribute__((always_inline)) {
-    u32x4 ah[FM], al[FM], bh[FN], bl[FN];
-#pragma unroll
-    for(int i = 0; i < FM; ++i) {
-      float x[8];
-      direct_frag(sa, t.a.kmajor, wm * FM + i, lane, x);
-      split_chunk<NHL>(x, ah[i], al[i]);
-    }
+  // A stage, software-pipelined inside the wave (a wave issues in order, so
+  // VALU work placed between MFMAs runs in their shadow): the B fragments and
+  // A fragment 0 are split up front; then for each A fragment i its MFMAs,
+  // each followed by one pair of A fragment i+1's split, while the LDS reads
+  // of fragment i+2 are in flight.  sched_barrier(0) keeps that order.  Per
+  // accumulator the products stay lo*hi, hi*lo, hi*hi, so results are those
+  // of the unpipelined order.
+  auto contract = [&](const u32x4* sa, const u32x4* sb, auto ak, auto bk) __attribute__((always_inline)) {
+    constexpr bool AK = decltype(ak)::value, BKM = decltype(bk)::value;
+    constexpr int MPF = NHL == 2 ? 3 * FN : FN;  // MFMAs per A fragment
+    constexpr int PPM = (4 + MPF - 1) / MPF;     // split pairs after each MFMA
+    u32x4 bh[FN], bl[FN];
+    float xa[FM][8];
+    u32x4 ah[FM], al[FM];
 #pragma unroll
     for(int j = 0; j < FN; ++j) {
       float x[8];
-      direct_frag(sb, t.b.kmajor, wn * FN + j, lane, x);
+      direct_frag<BKM>(sb, wn * FN + j, lane, x);
       split_chunk<NHL>(x, bh[j], bl[j]);
     }
-    if(NHL == 2) {
-#pragma unroll
-      for(int i = 0; i < FM; ++i)
-#pragma unroll
-        for(int j = 0; j < FN; ++j)
+    direct_frag<AK>(sa, wm * FM, lane, xa[0]);
+    if constexpr(FM > 1) direct_frag<AK>(sa, wm * FM + 1, lane, xa[1]);
+    split_chunk<NHL>(xa[0], ah[0], al[0]);
+    __builtin_amdgcn_sched_barrier(0);
+    static_for<FM>([&](auto ii) __attribute__((always_inline)) {
+      constexpr int i = decltype(ii)::value;
+      if constexpr(i + 2 < FM) direct_frag<AK>(sa, wm * FM + i + 2, lane, xa[i + 2]);
+      static_for<MPF>([&](auto mm) __attribute__((always_inline)) {
+        constexpr int m = decltype(mm)::value;
+        constexpr int prod = NHL == 2 ? m / FN : 2, j = m % FN;
+        if constexpr(prod == 0)
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(al[i]), as_bf16x8(bh[j]), acc[i][j], 0, 0, 0);
-#pragma unroll
-      for(int i = 0; i < FM; ++i)
-#pragma unroll
-        for(int j = 0; j < FN; ++j)
+        else if constexpr(prod == 1)
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(ah[i]), as_bf16x8(bl[j]), acc[i][j], 0, 0, 0);
-    }
+        else
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(ah[i]), as_bf16x8(bh[j]), acc[i][j], 0, 0, 0);
+        if constexpr(i + 1 < FM) {
 #pragma unroll
-    for(int i = 0; i < FM; ++i)
-#pragma unroll
-      for(int j = 0; j < FN; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(ah[i]), as_bf16x8(bh[j]), acc[i][j], 0, 0, 0);
+          for(int q = 0; q < PPM; ++q) {
+            const int p = m * PPM + q;
+            if(p < 4) {
+              uint32_t hw, lw;
+              split_pair<NHL>(xa[i + 1], p, hw, lw);
+              ah[i + 1][p] = hw;
+              al[i + 1][p] = lw;
+            }
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      });
+    });
   };
 
+  auto k_loop = [&](auto ak, auto bk) __attribute__((always_inline)) {
 #pragma unroll
-  for(int p = 0; p < NB - 1; ++p)
-    if(p < steps) issue(p);
-  for(int s = 0; s < steps; ++s) {
-    const int after = steps - 1 - s;
-    if(after >= NB - 2) vm_wait<(NB - 2) * PER>();
-    else if(NB > 3 && after == 1) vm_wait<PER>();
-    else lds_dma_wait();
-    __syncthreads();
-    if(s + NB - 1 < steps) issue(s + NB - 1);
-    const u32x4* sa = lds + (s % NB) * (SA + SB);
-    if(active) contract(sa, sa + SA);
+    for(int p = 0; p < NB - 1; ++p)
+      if(p < steps) issue(p);
+    for(int s = 0; s < steps; ++s) {
+      const int after = steps - 1 - s;
+      if(after >= NB - 2) vm_wait<(NB - 2) * PER>();
+      else if(NB > 3 && after == 1) vm_wait<PER>();
+      else lds_dma_wait();
+      __syncthreads();
+      if(s + NB - 1 < steps) issue(s + NB - 1);
+      const u32x4* sa = lds + (s % NB) * (SA + SB);
+      if(active) contract(sa, sa + SA, ak, bk);
+    }
+  };
+  // one instance of the loop per pair of operand layouts
+  using T = std::true_type;
+  using F = std::false_type;
+  if(t.a.kmajor) {
+    if(t.b.kmajor) k_loop(T{}, T{});
+    else k_loop(T{}, F{});
+  } else {
+    if(t.b.kmajor) k_loop(F{}, T{});
+    else k_loop(F{}, F{});
   }
 
   if(active) static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
