@@ -103,10 +103,11 @@ def calibrate(torch, h, reps=5):
     return res
 
 
-def grouped_workload(torch, mfx, h, steps, warmup, seed=1):
-    """512 independent problems (M, N, K log-uniform on [64, 2048], multiples of
-    16, from `seed`) plus an 8-long chain of 2048 x 2048 matrices, as one
-    grouped launch (autotuned tile policy) captured in a hipGraph, split-BF16."""
+def grouped_lists(torch, mfx, h, seed=1):
+    """The grouped workload's buffers and problem list: 512 independent
+    problems (M, N, K log-uniform on [64, 2048], multiples of 16, from `seed`)
+    plus the 7 products of an 8-long chain of 2048 x 2048 matrices with their
+    dependencies."""
     import numpy as np
     rng = np.random.default_rng(seed)
     dev = "cuda"
@@ -137,6 +138,13 @@ def grouped_workload(torch, mfx, h, steps, warmup, seed=1):
         probs.append(mfx.problem(x, y, out))
         flop += 2.0 * n ** 3
     torch.cuda.synchronize()
+    return probs, deps, mats, chain, inter, pairs, base, flop
+
+
+def grouped_workload(torch, mfx, h, steps, warmup, seed=1):
+    """The grouped workload (grouped_lists) as one grouped launch (autotuned
+    tile policy) captured in a hipGraph, split-BF16."""
+    probs, deps, mats, chain, inter, pairs, base, flop = grouped_lists(torch, mfx, h, seed)
 
     log("grouped: autotuning tile policy ...")
     policy = h.autotune_grouped(probs, deps, prec="split-bf16")
@@ -584,6 +592,9 @@ def main():
     ap.add_argument("--profile-pass", action="store_true",
                     help="short pass for counter collection: one split-BF16, FP32, single-BF16 and FP8-block gemm "
                          "at --n, plus a 1 GiB device copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters")
+    ap.add_argument("--profile-grouped", action="store_true",
+                    help="short pass for counter collection on the grouped workload: four eager launches of its "
+                         "512 + chain list as one persistent kernel (tile policy 29), split-BF16")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the dense-contraction path is built and measured on a single "
@@ -603,6 +614,18 @@ def main():
     h.fill_uniform(C0, 1, 2)
     alpha, beta = 1.5, 1.2
     h.reserve_workspace(mfx.load().mfx_gemm_workspace_size(mfx.SPLIT_BF16, n, n, n))
+
+    if args.profile_grouped:
+        lists = grouped_lists(torch, mfx, h)  # keeps the buffers alive
+        probs, deps = lists[0], lists[1]
+        h.force_grouped_policy(29)
+        for i in range(4):
+            h.grouped(probs, deps, prec="split-bf16")
+            torch.cuda.synchronize()
+            log(f"profile pass: grouped launch {i} done")
+        h.grouped_status()
+        h.close()
+        return
 
     if args.profile_pass:
         C = C0.clone()
