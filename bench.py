@@ -138,13 +138,13 @@ def grouped_lists(torch, mfx, h, seed=1):
         probs.append(mfx.problem(x, y, out))
         flop += 2.0 * n ** 3
     torch.cuda.synchronize()
-    return probs, deps, mats, chain, inter, pairs, base, flop
+    return probs, deps, mats, chain, inter, pairs, base, flop, dims
 
 
 def grouped_workload(torch, mfx, h, steps, warmup, seed=1):
     """The grouped workload (grouped_lists) as one grouped launch (autotuned
     tile policy) captured in a hipGraph, split-BF16."""
-    probs, deps, mats, chain, inter, pairs, base, flop = grouped_lists(torch, mfx, h, seed)
+    probs, deps, mats, chain, inter, pairs, base, flop, dims = grouped_lists(torch, mfx, h, seed)
 
     log("grouped: autotuning tile policy ...")
     policy = h.autotune_grouped(probs, deps, prec="split-bf16")
