@@ -322,6 +322,13 @@ def measure_configs(torch, mfx, h, reps=3):
     T, D, E, F, G = (torch.empty((n, n), device=dev) for _ in range(5))
     for prec in ("fp32", "split-bf16"):
         for grouped in (False, True):
+            if grouped:
+                # the autotuner's policy for these lists, as for the grouped workload
+                h.autotune_grouped([mfx.problem(A, B, T, alpha=1.5), mfx.problem(T, Cm, D, alpha=1.0, beta=1.2)],
+                                   [(1, 0)], prec=prec)
+                h.autotune_grouped([mfx.problem(A, B, E), mfx.problem(Cm, D0, F), mfx.problem(E, F, G)],
+                                   [(2, 0), (2, 1)], prec=prec)
+
             def mm2():
                 if grouped:
                     h.grouped([mfx.problem(A, B, T, alpha=1.5), mfx.problem(T, Cm, D, alpha=1.0, beta=1.2)],
@@ -385,7 +392,15 @@ def measure_configs(torch, mfx, h, reps=3):
     level = [m.double() for m in mats]
     while len(level) > 1:
         level = [level[i] @ level[i + 1] for i in range(0, len(level), 2)]
+    # mfx_chain's own list (balanced tree: four products, two, one), autotuned
+    # under the same key as the chain call's
+    outs = scratch[:4] + scratch[4:6] + [res]
+    ins = [(mats[0], mats[1]), (mats[2], mats[3]), (mats[4], mats[5]), (mats[6], mats[7]),
+           (scratch[0], scratch[1]), (scratch[2], scratch[3]), (scratch[4], scratch[5])]
+    chain_list = [mfx.problem(x, y, o) for (x, y), o in zip(ins, outs)]
+    chain_deps = [(4, 0), (4, 1), (5, 2), (5, 3), (6, 4), (6, 5)]
     for prec in ("fp32", "split-bf16"):
+        h.autotune_grouped(chain_list, chain_deps, prec=prec)
         chn = lambda: h.chain(mats, scratch, res, prec=prec)
         ms = _time_ms(torch, chn, reps)
         rec(f"matmulchain_8x2048_{prec.replace('-', '')}", 7 * 2.0 * n ** 3, ms, h.rel_frobenius(res, level[0]), chn)
