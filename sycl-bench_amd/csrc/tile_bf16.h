@@ -651,8 +651,16 @@ __device__ __forceinline__ void bf16_tile_direct(const DirectTileArgs& t, int tm
   // of fragment i+2 are in flight.  sched_barrier(0) keeps that order.  Per
   // accumulator the products stay lo*hi, hi*lo, hi*hi, so results are those
   // of the unpipelined order.
-  auto contract = [&](const u32x4* sa, const u32x4* sb, auto ak, auto bk) __attribute__((always_inline)) {
-    constexpr bool AK = decltype(ak)::value, BKM = decltype(bk)::value;
+  // A wave whose sub-tile the problem covers only in part (PARTIAL) also
+  // skips, within it, the 32-row A blocks (i >= va) and 32-column B blocks
+  // (j >= vb) that miss the problem: their LDS reads, splits and MFMAs.  That
+  // copy of the loop runs only for such waves; a wholly covered wave runs the
+  // loop without those tests.
+  const int va = active ? min(FM, (t.ep.M - (tm * BM + wm * WM) + 31) / 32) : 0;
+  const int vb = active ? min(FN, (t.ep.N - (tn * BN + wn * WN) + 31) / 32) : 0;
+  const bool whole = va == FM && vb == FN;
+  auto contract = [&](const u32x4* sa, const u32x4* sb, auto ak, auto bk, auto part) __attribute__((always_inline)) {
+    constexpr bool AK = decltype(ak)::value, BKM = decltype(bk)::value, PARTIAL = decltype(part)::value;
     constexpr int MPF = NHL == 2 ? 3 * FN : FN;  // MFMAs per A fragment
     constexpr int PPM = (4 + MPF - 1) / MPF;     // split pairs after each MFMA
     u32x4 bh[FN], bl[FN];
@@ -660,35 +668,44 @@ __device__ __forceinline__ void bf16_tile_direct(const DirectTileArgs& t, int tm
     u32x4 ah[FM], al[FM];
 #pragma unroll
     for(int j = 0; j < FN; ++j) {
-      float x[8];
-      direct_frag<BKM>(sb, wn * FN + j, lane, x);
-      split_chunk<NHL>(x, bh[j], bl[j]);
+      if(!PARTIAL || j < vb) {
+        float x[8];
+        direct_frag<BKM>(sb, wn * FN + j, lane, x);
+        split_chunk<NHL>(x, bh[j], bl[j]);
+      }
     }
     direct_frag<AK>(sa, wm * FM, lane, xa[0]);
-    if constexpr(FM > 1) direct_frag<AK>(sa, wm * FM + 1, lane, xa[1]);
+    if constexpr(FM > 1)
+      if(!PARTIAL || 1 < va) direct_frag<AK>(sa, wm * FM + 1, lane, xa[1]);
     split_chunk<NHL>(xa[0], ah[0], al[0]);
     __builtin_amdgcn_sched_barrier(0);
     static_for<FM>([&](auto ii) __attribute__((always_inline)) {
       constexpr int i = decltype(ii)::value;
-      if constexpr(i + 2 < FM) direct_frag<AK>(sa, wm * FM + i + 2, lane, xa[i + 2]);
+      if(PARTIAL && i >= va) return;
+      if constexpr(i + 2 < FM)
+        if(!PARTIAL || i + 2 < va) direct_frag<AK>(sa, wm * FM + i + 2, lane, xa[i + 2]);
       static_for<MPF>([&](auto mm) __attribute__((always_inline)) {
         constexpr int m = decltype(mm)::value;
         constexpr int prod = NHL == 2 ? m / FN : 2, j = m % FN;
-        if constexpr(prod == 0)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(al[i]), as_bf16x8(bh[j]), acc[i][j], 0, 0, 0);
-        else if constexpr(prod == 1)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(ah[i]), as_bf16x8(bl[j]), acc[i][j], 0, 0, 0);
-        else
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(ah[i]), as_bf16x8(bh[j]), acc[i][j], 0, 0, 0);
+        if(!PARTIAL || j < vb) {
+          if constexpr(prod == 0)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(al[i]), as_bf16x8(bh[j]), acc[i][j], 0, 0, 0);
+          else if constexpr(prod == 1)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(ah[i]), as_bf16x8(bl[j]), acc[i][j], 0, 0, 0);
+          else
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(ah[i]), as_bf16x8(bh[j]), acc[i][j], 0, 0, 0);
+        }
         if constexpr(i + 1 < FM) {
+          if(!PARTIAL || i + 1 < va) {
 #pragma unroll
-          for(int q = 0; q < PPM; ++q) {
-            const int p = m * PPM + q;
-            if(p < 4) {
-              uint32_t hw, lw;
-              split_pair<NHL>(xa[i + 1], p, hw, lw);
-              ah[i + 1][p] = hw;
-              al[i + 1][p] = lw;
+            for(int q = 0; q < PPM; ++q) {
+              const int p = m * PPM + q;
+              if(p < 4) {
+                uint32_t hw, lw;
+                split_pair<NHL>(xa[i + 1], p, hw, lw);
+                ah[i + 1][p] = hw;
+                al[i + 1][p] = lw;
+              }
             }
           }
         }
@@ -697,7 +714,7 @@ __device__ __forceinline__ void bf16_tile_direct(const DirectTileArgs& t, int tm
     });
   };
 
-  auto k_loop = [&](auto ak, auto bk) __attribute__((always_inline)) {
+  auto k_loop = [&](auto ak, auto bk, auto part) __attribute__((always_inline)) {
 #pragma unroll
     for(int p = 0; p < NB - 1; ++p)
       if(p < steps) issue(p);
@@ -709,18 +726,24 @@ __device__ __forceinline__ void bf16_tile_direct(const DirectTileArgs& t, int tm
       __syncthreads();
       if(s + NB - 1 < steps) issue(s + NB - 1);
       const u32x4* sa = lds + (s % NB) * (SA + SB);
-      if(active) contract(sa, sa + SA, ak, bk);
+      if(active) contract(sa, sa + SA, ak, bk, part);
     }
   };
-  // one instance of the loop per pair of operand layouts
+  // one instance of the loop per pair of operand layouts and coverage (every
+  // instance passes the same barriers, so the waves of a workgroup may run
+  // different ones)
   using T = std::true_type;
   using F = std::false_type;
+  auto by_cover = [&](auto ak, auto bk) __attribute__((always_inline)) {
+    if(whole) k_loop(ak, bk, F{});
+    else k_loop(ak, bk, T{});
+  };
   if(t.a.kmajor) {
-    if(t.b.kmajor) k_loop(T{}, T{});
-    else k_loop(T{}, F{});
+    if(t.b.kmajor) by_cover(T{}, T{});
+    else by_cover(T{}, F{});
   } else {
-    if(t.b.kmajor) k_loop(F{}, T{});
-    else k_loop(F{}, F{});
+    if(t.b.kmajor) by_cover(F{}, T{});
+    else by_cover(F{}, F{});
   }
 
   if(active) static_for<FM * FN>([&](auto ij) __attribute__((always_inline)) {
